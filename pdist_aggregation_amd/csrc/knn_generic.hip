@@ -1,0 +1,173 @@
+// Generic fused distance + candidate-selection kernel (SIMT FMA, any d, any
+// k' up to kMaxCandidates, float or double).
+//
+// Role in the reference: the GEMM + heap-push loop of _k_closest_on_chunk
+// (src/_pdist_aggregation.pyx:192-232) and the SYCL kern_pwdist2/kern_topk
+// pair (sycl_ext/parallel_knn.hpp:199-289, :397-466), without ever storing
+// the query x train distance tile in HBM.
+//
+// Workgroup (qb, s) owns 64 queries and the train rows of split s.  Per step
+// it stages a 64-row train slice and the query slice through LDS (16 features
+// at a time) and every thread accumulates a 4 x 4 block of dot products.  The
+// reduced distance v = |y|^2 - 2 x.y (the reference's "reduced distance",
+// :225-232) is compared with the query's threshold tau; survivors are
+// appended to the query's candidate list in global scratch.  When a list
+// could overflow on the next step, the workgroup sorts it by (v, index),
+// keeps the best k' and lowers tau to the k'-th value.  tau is also shared
+// between the workgroups of one query through an atomicMin on an ordered key,
+// so every split filters with the best threshold found anywhere.
+#include "knn_common.hpp"
+#include "knn_kernels.hpp"
+
+namespace knn {
+
+namespace {
+
+template <class T>
+__device__ void compact_lists(const SearchArgs<T>& a, T* cd, int* ci, T* tau_s, int* cnt_s,
+                              int64_t q0, int min_count, T* sd, int* si) {
+  __syncthreads();
+  for (int qq = 0; qq < kGenQ; ++qq) {
+    const int n = cnt_s[qq];  // uniform: LDS value written before a barrier
+    if (n <= min_count) continue;
+    const int P = next_pow2(n);
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+      const bool ok = e < n;
+      sd[e] = ok ? cd[(size_t)qq * a.cap + e] : pos_inf<T>();
+      si[e] = ok ? ci[(size_t)qq * a.cap + e] : INT_MAX;
+    }
+    block_bitonic_sort(sd, si, P);
+    const int m = n < a.kc ? n : a.kc;
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+      cd[(size_t)qq * a.cap + e] = sd[e];
+      ci[(size_t)qq * a.cap + e] = si[e];
+    }
+    if (threadIdx.x == 0) {
+      cnt_s[qq] = m;
+      if (m == a.kc) {
+        const T kth = sd[a.kc - 1];
+        if (kth < tau_s[qq]) tau_s[qq] = kth;
+        atomicMin(&a.tau[q0 + qq], ordered_key((double)kth));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> a) {
+  __shared__ T xs[kGenK][kGenQ + 1];
+  __shared__ T ys[kGenK][kGenY + 1];
+  __shared__ T tau_s[kGenQ];
+  __shared__ int cnt_s[kGenQ];
+  __shared__ int flag_s;
+  extern __shared__ __align__(16) unsigned char dyn_smem[];
+  T* sd = reinterpret_cast<T*>(dyn_smem);
+  int* si = reinterpret_cast<int*>(dyn_smem + (size_t)a.cap * sizeof(T));
+
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int64_t q0 = (int64_t)blockIdx.x * kGenQ;
+  const int64_t ybeg = (int64_t)blockIdx.y * a.rows_per_split;
+  const int64_t yend = ybeg + a.rows_per_split < a.n_t ? ybeg + a.rows_per_split : a.n_t;
+  const size_t blk = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  T* cd = a.cand_d + blk * kGenQ * a.cap;
+  int* ci = a.cand_i + blk * kGenQ * a.cap;
+  const int64_t n_q_here = a.n_q - q0 < kGenQ ? a.n_q - q0 : kGenQ;
+
+  if (tid < kGenQ) {
+    tau_s[tid] = pos_inf<T>();
+    cnt_s[tid] = 0;
+  }
+  if (tid == 0) flag_s = 0;
+  __syncthreads();
+
+  int step = 0;
+  for (int64_t y0 = ybeg; y0 < yend; y0 += kGenY, ++step) {
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+
+    for (int p0 = 0; p0 < a.d; p0 += kGenK) {
+      for (int e = tid; e < kGenQ * kGenK; e += kThreads) {
+        const int r = e / kGenK, c = e % kGenK;
+        const int64_t q = q0 + r;
+        const int p = p0 + c;
+        xs[c][r] = (r < n_q_here && p < a.d) ? a.X[q * a.d + p] : T(0);
+        const int64_t y = y0 + r;
+        ys[c][r] = (y < yend && p < a.d) ? a.Y[y * a.d + p] : T(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < kGenK; ++c) {
+        T xv[4], yv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xv[i] = xs[c][tx + 16 * i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) yv[j] = ys[c][ty + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = fma(xv[i], yv[j], acc[i][j]);
+      }
+      __syncthreads();
+    }
+
+    // Filter: append every reduced distance <= tau to the query's list.
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int qq = tx + 16 * i;
+      if (qq >= n_q_here) continue;
+      const T tq = tau_s[qq];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t y = y0 + ty + 16 * j;
+        if (y >= yend) continue;
+        const T v = a.ynorm[y] - T(2) * acc[i][j];
+        if (v <= tq) {
+          const int pos = atomicAdd(&cnt_s[qq], 1);
+          cd[(size_t)qq * a.cap + pos] = v;
+          ci[(size_t)qq * a.cap + pos] = (int)y;
+          if (pos + 1 > a.cap - kGenY) flag_s = 1;
+        }
+      }
+    }
+    __syncthreads();
+    if (flag_s) {
+      compact_lists(a, cd, ci, tau_s, cnt_s, q0, a.cap - kGenY, sd, si);
+      if (tid == 0) flag_s = 0;
+    }
+    if ((step & 7) == 7 && tid < n_q_here) {
+      const T g = (T)from_ordered_key(__atomic_load_n(&a.tau[q0 + tid], __ATOMIC_RELAXED));
+      if (g < tau_s[tid]) tau_s[tid] = g;
+    }
+    __syncthreads();
+  }
+
+  // Final: every list sorted and cut to k'; written to this split's slot.
+  compact_lists(a, cd, ci, tau_s, cnt_s, q0, 0, sd, si);
+  for (int e = tid; e < kGenQ * a.kc; e += kThreads) {
+    const int qq = e / a.kc, j = e % a.kc;
+    if (qq >= n_q_here) continue;
+    const int n = cnt_s[qq];
+    const size_t o = ((size_t)(q0 + qq) * a.n_splits + blockIdx.y) * a.kc + j;
+    a.part_d[o] = j < n ? cd[(size_t)qq * a.cap + j] : pos_inf<T>();
+    a.part_i[o] = j < n ? ci[(size_t)qq * a.cap + j] : -1;
+  }
+}
+
+}  // namespace
+
+template <class T>
+void launch_generic_search(const SearchArgs<T>& a, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(a.n_q, kGenQ), (unsigned)a.n_splits);
+  const size_t lds = (size_t)a.cap * (sizeof(T) + sizeof(int));
+  hipLaunchKernelGGL(generic_search_kernel<T>, grid, dim3(kThreads), lds, s, a);
+}
+
+template void launch_generic_search<float>(const SearchArgs<float>&, hipStream_t);
+template void launch_generic_search<double>(const SearchArgs<double>&, hipStream_t);
+
+}  // namespace knn

@@ -1,0 +1,88 @@
+// Internal launch interface between the scheduler (knn_capi.hip) and the
+// kernels.  Every pointer is a device pointer; every launch is asynchronous on
+// `stream`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace knn {
+
+// Geometry of the generic (SIMT FMA) fused distance + candidate kernel.
+constexpr int kGenQ = 64;  // queries per workgroup
+constexpr int kGenY = 64;  // train rows per step
+constexpr int kGenK = 16;  // feature slice staged in LDS per step
+
+// Largest candidate list (k') a query may keep, and the largest merged
+// (splits x k') list the merge kernel sorts in LDS.
+constexpr int kMaxCandidates = 1984;
+constexpr int kMaxMerge = 2048;
+
+// Per-call scalars the kernels share (lives in the workspace).
+struct DeviceScalars {
+  unsigned long long max_ynorm_key;  // ordered key of max_j |y_j|^2 (as used by the search)
+  unsigned long long max_yerr_key;   // ordered key of max_j |y_j - q(y_j)| (low precision)
+  unsigned int n_uncertified;
+  unsigned int pad;
+};
+
+// Everything the fused search kernels need.  T is the element type of the
+// search arithmetic inputs (float or double); candidate values are T.
+template <class T>
+struct SearchArgs {
+  const T* X;        // queries, n_q x d row-major
+  const T* Y;        // train rows, n_t x d row-major
+  const T* ynorm;    // |y_j|^2, n_t
+  int64_t n_q, n_t;
+  int d;
+  int kc;                  // candidates kept per query (k')
+  int cap;                 // per-query capacity of the candidate scratch (power of two)
+  int n_splits;
+  int64_t rows_per_split;  // multiple of the kernel's row step
+  T* cand_d;               // [blocks][queries per block][cap]
+  int* cand_i;
+  unsigned long long* tau;  // [n_q] ordered key of the best known k'-th value
+  T* part_d;                // [n_q][n_splits][kc] sorted candidate lists
+  int* part_i;
+};
+
+template <class T>
+void launch_fill_keys(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+
+template <class T>
+void launch_row_norms(const T* Y, int64_t n, int d, T* out, DeviceScalars* sc, hipStream_t s);
+
+template <class T>
+void launch_generic_search(const SearchArgs<T>& a, hipStream_t s);
+
+// Merge the per-split candidate lists, re-rank them with exact FP64
+// direct-difference distances from the original rows, keep k.
+template <class T>
+struct MergeArgs {
+  const T* X;  // original queries (n_q x d)
+  const T* Y;  // original train rows (n_t x d)
+  int64_t n_q, n_t;
+  int d;
+  const T* part_d;
+  const int* part_i;
+  int n_splits, kc, k;
+  int squared;  // output squared distances instead of Euclidean
+  // Certification: the approximate values were computed on vectors xq, yq
+  // with |x - xq| <= xerr[i] (nullptr: 0), |y - yq| <= max_yerr, rows norms
+  // |xq|^2 = xnorm[i] (nullptr: computed from X) and |yq|^2 <= max_ynorm,
+  // with arithmetic of unit roundoff `unit_roundoff` over `acc_terms` terms.
+  const float* xnorm;
+  const float* xerr;
+  double unit_roundoff;
+  int acc_terms;
+  DeviceScalars* sc;
+  T* dist_out;        // n_q x k
+  int64_t* idx_out;   // n_q x k
+};
+
+template <class T>
+void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s);
+
+size_t merge_lds_bytes(int n_splits, int kc, size_t elem);
+
+}  // namespace knn

@@ -1,0 +1,184 @@
+// Row norms, split-list merge, exact re-rank and certification.
+//
+// Reference counterparts:
+//  * row norms: np.einsum('ij,ij->i', X_train, X_train)
+//    (src/_pdist_aggregation.pyx:508);
+//  * merging per-split lists: the GIL-serialised heap merge of
+//    _parallel_knn_on_X_train (:420-430) and the SYCL kern_aggregate
+//    (sycl_ext/parallel_knn.hpp:474-509);
+//  * exact distances of the chosen neighbours: _compute_exact_distances
+//    (:450-487).  Unlike the reference, rows are re-sorted on the exact
+//    distance with the lower index winning ties.
+#include "knn_common.hpp"
+#include "knn_kernels.hpp"
+
+namespace knn {
+
+namespace {
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <class T>
+__global__ void fill_keys_kernel(unsigned long long* p, int64_t n, unsigned long long v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// One wave per row: |y_j|^2 accumulated in T (the precision the search uses)
+// and the maximum over all rows as an ordered key.
+template <class T>
+__global__ __launch_bounds__(kThreads) void row_norms_kernel(const T* __restrict__ Y, int64_t n,
+                                                             int d, T* __restrict__ out,
+                                                             DeviceScalars* sc) {
+  __shared__ T wmax[kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T best = T(0);
+  for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
+    T s = T(0);
+    for (int p = lane; p < d; p += 64) {
+      const T v = Y[r * d + p];
+      s = fma(v, v, s);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) out[r] = s;
+    best = s > best ? s : best;
+  }
+  if (lane == 0) wmax[w] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T m = wmax[0];
+    for (int i = 1; i < kThreads / 64; ++i) m = wmax[i] > m ? wmax[i] : m;
+    atomicMax(&sc->max_ynorm_key, ordered_key((double)m));
+  }
+}
+
+// One workgroup per query.
+template <class T>
+__global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char dyn_smem[];
+  const int total = a.n_splits * a.kc;
+  const int P = next_pow2(total);
+  const int KP = next_pow2(a.kc);
+  const size_t o_si = align16((size_t)P * sizeof(T));
+  const size_t o_ed = align16(o_si + (size_t)P * sizeof(int));
+  const size_t o_ei = o_ed + (size_t)KP * sizeof(double);
+  T* sd = reinterpret_cast<T*>(dyn_smem);
+  int* si = reinterpret_cast<int*>(dyn_smem + o_si);
+  double* ed = reinterpret_cast<double*>(dyn_smem + o_ed);
+  int* ei = reinterpret_cast<int*>(dyn_smem + o_ei);
+  __shared__ double xnorm_s;
+
+  const int64_t q = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const T* pd = a.part_d + (size_t)q * total;
+  const int* pi = a.part_i + (size_t)q * total;
+  for (int e = tid; e < P; e += kThreads) {
+    const bool ok = e < total && pi[e] >= 0;
+    sd[e] = ok ? pd[e] : pos_inf<T>();
+    si[e] = ok ? pi[e] : INT_MAX;
+  }
+  block_bitonic_sort(sd, si, P);  // ends with a barrier
+
+  // Exact FP64 direct-difference distances of the best k' candidates,
+  // one wave per candidate, lanes across features.
+  const T* xr = a.X + q * a.d;
+  for (int j = w; j < KP; j += kThreads / 64) {
+    const int id = j < a.kc ? si[j] : INT_MAX;
+    double s = 0.0;
+    if (id != INT_MAX) {
+      const T* yr = a.Y + (int64_t)id * a.d;
+      for (int p = lane; p < a.d; p += 64) {
+        const double df = (double)xr[p] - (double)yr[p];
+        s = fma(df, df, s);
+      }
+      s = wave_sum(s);
+    }
+    if (lane == 0) {
+      ed[j] = id != INT_MAX ? s : __builtin_huge_val();
+      ei[j] = id;
+    }
+  }
+  if (w == 0) {
+    double s = 0.0;
+    for (int p = lane; p < a.d; p += 64) {
+      const double v = (double)xr[p];
+      s = fma(v, v, s);
+    }
+    s = wave_sum(s);
+    if (lane == 0) xnorm_s = s;
+  }
+  block_bitonic_sort(ed, ei, KP);
+
+  for (int j = tid; j < a.k; j += kThreads) {
+    const bool ok = j < a.kc && ei[j] != INT_MAX;
+    const double e = ok ? ed[j] : __builtin_huge_val();
+    a.idx_out[q * a.k + j] = ok ? (int64_t)ei[j] : -1;
+    a.dist_out[q * a.k + j] = (T)(a.squared ? e : sqrt(e));
+  }
+
+  // Certification: every train row left out of the k' candidates has an
+  // approximate value >= a_kc (the k'-th).  Its true distance is then at
+  // least sqrt(max(0, a_kc + |xq|^2 - delta)) - xerr - max_yerr; if that
+  // exceeds the k-th exact distance the k returned rows are the exact answer.
+  if (tid == 0) {
+    bool certified = a.kc >= a.n_t;
+    if (!certified && a.kc >= a.k && si[a.kc - 1] != INT_MAX) {
+      const double akc = (double)sd[a.kc - 1];
+      const double nx = a.xnorm ? (double)a.xnorm[q] : xnorm_s;
+      const double ny = from_ordered_key(a.sc->max_ynorm_key);
+      const double n = (double)a.acc_terms;
+      const double gamma = n * a.unit_roundoff / (1.0 - n * a.unit_roundoff);
+      const double delta = gamma * (ny + 2.0 * sqrt(nx * ny) + fabs(akc) + nx);
+      const double ex = a.xerr ? (double)a.xerr[q] : 0.0;
+      const double ey = a.xerr ? from_ordered_key(a.sc->max_yerr_key) : 0.0;
+      const double lo = akc + nx - delta;
+      const double lower = sqrt(lo > 0.0 ? lo : 0.0) - ex - ey;
+      certified = lower > sqrt(ed[a.k - 1]);
+    }
+    if (!certified) atomicAdd(&a.sc->n_uncertified, 1u);
+  }
+}
+
+}  // namespace
+
+template <class T>
+void launch_fill_keys(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = ceil_div(n, kThreads) < 4096 ? ceil_div(n, kThreads) : 4096;
+  hipLaunchKernelGGL(fill_keys_kernel<T>, dim3((unsigned)blocks), dim3(kThreads), 0, s, p, n, v);
+}
+
+template <class T>
+void launch_row_norms(const T* Y, int64_t n, int d, T* out, DeviceScalars* sc, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = ceil_div(n, 4) < 8192 ? ceil_div(n, 4) : 8192;
+  hipLaunchKernelGGL(row_norms_kernel<T>, dim3((unsigned)blocks), dim3(kThreads), 0, s, Y, n, d,
+                     out, sc);
+}
+
+size_t merge_lds_bytes(int n_splits, int kc, size_t elem) {
+  const size_t P = (size_t)next_pow2(n_splits * kc);
+  const size_t KP = (size_t)next_pow2(kc);
+  return align16(align16(P * elem) + P * sizeof(int)) + KP * (sizeof(double) + sizeof(int));
+}
+
+template <class T>
+void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s) {
+  if (a.n_q <= 0) return;
+  const size_t lds = merge_lds_bytes(a.n_splits, a.kc, sizeof(T));
+  hipLaunchKernelGGL(merge_rerank_kernel<T>, dim3((unsigned)a.n_q), dim3(kThreads), lds, s, a);
+}
+
+template void launch_fill_keys<float>(unsigned long long*, int64_t, unsigned long long, hipStream_t);
+template void launch_row_norms<float>(const float*, int64_t, int, float*, DeviceScalars*, hipStream_t);
+template void launch_row_norms<double>(const double*, int64_t, int, double*, DeviceScalars*, hipStream_t);
+template void launch_merge_rerank<float>(const MergeArgs<float>&, hipStream_t);
+template void launch_merge_rerank<double>(const MergeArgs<double>&, hipStream_t);
+
+}  // namespace knn

@@ -1,0 +1,166 @@
+"""``parallel_knn``: the reference's Python entry point, on the MI355X kernels.
+
+Mirrors ``parallel_knn`` of the reference's Cython module
+(src/_pdist_aggregation.pyx:492-540): same positional order (train set
+first), same keywords and defaults, same return shape
+``((distances, indices), n_parallel_chunks)`` or ``(indices,
+n_parallel_chunks)``, Euclidean distances by default.  Extra keyword-only
+arguments select the candidate precision, the oversampling, squared
+distances and the stream.
+
+Inputs may be numpy arrays (copied to the GPU and back) or torch tensors
+already on the GPU (no host copies; results come back as GPU tensors).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import _capi
+
+CHUNK_SIZE = 256  # src/_pdist_aggregation.pyx:24 (default of parallel_knn)
+
+
+def effective_n_threads():
+    """The reference's _openmp_effective_n_threads (:41-49).
+
+    OMP_NUM_THREADS wins if set; otherwise min(CPU count, physical cores).
+    It only feeds the "auto" strategy rule and n_parallel_chunks.
+    """
+    env = os.getenv("OMP_NUM_THREADS")
+    if env:
+        try:
+            return max(1, int(env.split(",")[0]))
+        except ValueError:
+            pass
+    n = os.cpu_count() or 1
+    try:
+        from joblib import cpu_count
+
+        return max(1, min(n, cpu_count(only_physical_cores=True)))
+    except Exception:  # joblib absent: every CPU counts
+        return n
+
+
+def resolve_strategy(strategy, chunk_size, n_threads, n_test):
+    """The "auto" rule of src/_pdist_aggregation.pyx:511-515."""
+    if strategy == "auto":
+        return "chunk_on_test" if 4 * chunk_size * n_threads < n_test else "chunk_on_train"
+    return strategy
+
+
+def n_parallel_chunks(strategy, n_train, n_test, chunk_size, n_threads):
+    """Second return value of the reference's parallel_knn: the number of
+    chunks of the axis its strategy parallelises (:247-269, :330, :448)."""
+    s = resolve_strategy(strategy, chunk_size, n_threads, n_test)
+    n = n_test if s == "chunk_on_test" else n_train
+    per = min(n, max(20, int(chunk_size)))
+    if per <= 0:
+        return 0
+    full = n // per
+    return full + (n != full * per)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _as_device(a, name, torch, device):
+    """Return (tensor on GPU, came_from_numpy)."""
+    if isinstance(a, torch.Tensor):
+        if a.dim() != 2:
+            raise ValueError(f"{name} must be a 2-D array")
+        if not a.is_contiguous():
+            raise ValueError(f"{name} is not C-contiguous")
+        if a.device.type != "cuda":
+            return a.to(device), True
+        return a, False
+    arr = np.asarray(a)
+    if arr.ndim != 2:
+        raise ValueError(f"{name} must be a 2-D array")
+    if not arr.flags.c_contiguous:  # the reference's floating[:, ::1] memoryview
+        raise ValueError(f"{name} is not C-contiguous")
+    return torch.from_numpy(arr).to(device), True
+
+
+def parallel_knn(X_train, X_test, k, chunk_size=CHUNK_SIZE, strategy="auto",
+                 return_distance=False, *, squared=False, precision="exact",
+                 oversample=None, stream=None, return_stats=False, n_threads=None):
+    """k nearest rows of ``X_train`` for every row of ``X_test``.
+
+    Parameters follow src/_pdist_aggregation.pyx:492-499.  ``strategy`` and
+    ``chunk_size`` do not change the GPU schedule; they determine the
+    returned ``n_parallel_chunks`` exactly as the reference computes it.
+
+    Keyword-only extensions
+    -----------------------
+    squared : return squared distances (the SYCL backend's convention).
+    precision : "exact" (search in the input type), "bf16" or "fp8" (low
+        precision candidates, then exact re-rank; float32 inputs only).
+    oversample : k' candidates kept before the exact re-rank.
+    stream : a torch.cuda.Stream (default: the current stream).
+    return_stats : append a dict of call statistics to the return tuple.
+
+    Rows are sorted by exact distance; equal distances list the lower train
+    index first.
+    """
+    if strategy not in _capi.STRATEGY:
+        raise RuntimeError(f"strategy '{strategy}' not supported.")
+    if precision not in _capi.PRECISION:
+        raise ValueError(f"precision '{precision}' not supported")
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError("pdist_aggregation_amd needs a visible AMD GPU (no CPU fallback)")
+    lib = _capi.lib()
+    device = X_train.device if isinstance(X_train, torch.Tensor) and X_train.is_cuda \
+        else torch.device("cuda", torch.cuda.current_device())
+    Ytr, host_in = _as_device(X_train, "X_train", torch, device)
+    Xte, host_q = _as_device(X_test, "X_test", torch, device)
+    if Ytr.dtype != Xte.dtype or Ytr.dtype not in (torch.float32, torch.float64):
+        raise TypeError("X_train and X_test must both be float32 or both float64")
+    if Ytr.shape[1] != Xte.shape[1]:
+        raise ValueError("Dimensionality of test and train datasets must be the same")
+    k = int(k)
+    n_train, dim = Ytr.shape
+    n_test = Xte.shape[0]
+    if k <= 0:
+        raise ValueError("Number requested nearest neighbors must be non-zero")
+    if k > n_train:
+        raise ValueError(f"k (={k}) must not exceed the number of train rows (={n_train})")
+    is64 = Ytr.dtype == torch.float64
+    prec = _capi.PRECISION[precision]
+    ov = int(oversample) if oversample else 0
+    ws_bytes = lib.knn_argkmin_workspace_size(8 if is64 else 4, n_train, n_test, dim, k,
+                                              prec, ov)
+    if ws_bytes == 0:
+        _capi.check(_capi.KNN_EINVAL)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+    dist = torch.empty((n_test, k), dtype=Ytr.dtype, device=device)
+    idx = torch.empty((n_test, k), dtype=torch.int64, device=device)
+    s = stream if stream is not None else torch.cuda.current_stream(device)
+    stats = _capi.KnnStats()
+    fn = lib.knn_argkmin_f64 if is64 else lib.knn_argkmin_f32
+    threads = effective_n_threads() if n_threads is None else int(n_threads)
+    with torch.cuda.device(device):
+        # Passing stats makes the call wait for the GPU; only on request.
+        code = fn(Ytr.data_ptr(), n_train, Xte.data_ptr(), n_test, dim, k, int(chunk_size),
+                  _capi.STRATEGY[strategy], prec, ov, _capi.FLAG_SQUARED if squared else 0,
+                  threads, dist.data_ptr(), idx.data_ptr(), ws.data_ptr(), ws_bytes,
+                  ctypes.c_void_p(s.cuda_stream),
+                  ctypes.byref(stats) if return_stats else None)
+    _capi.check(code)
+    n_chunks = n_parallel_chunks(strategy, n_train, n_test, chunk_size, threads)
+    if not (isinstance(X_test, torch.Tensor) and X_test.is_cuda):
+        dist_o, idx_o = dist.cpu().numpy(), idx.cpu().numpy()
+    else:
+        dist_o, idx_o = dist, idx
+    out = ((dist_o, idx_o), n_chunks) if return_distance else (idx_o, n_chunks)
+    if return_stats:
+        st = stats.as_dict()
+        st["strategy_used"] = _capi.STRATEGY_NAME.get(st["strategy_used"], "?")
+        return out + (st,)
+    return out

@@ -1,0 +1,54 @@
+"""Parity assertions shared by the test modules."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle
+
+# Near-tie half-width, relative to max(1, |distance^2|).  FP32 GEMM-expansion
+# candidates can swap two train rows whose exact squared distances differ by
+# less than a few FP32 ulps of |x|^2 + |y|^2; such swaps are exempt (and
+# counted), everything else must match index for index.
+EPS_F32 = 2e-5
+EPS_F64 = 1e-10
+DIST_RTOL = 1e-5
+
+
+def assert_sorted_unique(idx, dist, n_train):
+    idx = np.asarray(idx)
+    dist = np.asarray(dist)
+    assert idx.min() >= 0 and idx.max() < n_train
+    assert np.all(np.diff(dist, axis=1) >= 0), "rows not sorted ascending"
+    s = np.sort(idx, axis=1)
+    assert np.all(s[:, 1:] != s[:, :-1]), "duplicate index in a row"
+    # equal distances list the lower index first
+    eq = np.diff(dist, axis=1) == 0
+    assert np.all(np.diff(idx, axis=1)[eq] > 0), "tie not broken by lower index"
+
+
+def assert_knn_parity(X_test, X_train, dist, idx, k, eps_rel, squared=True, dist_rtol=DIST_RTOL,
+                      ref=None):
+    """Compare an answer with the exact FP64 oracle.
+
+    Indices must be identical except at near-ties (|gap| <= eps_rel); distances
+    within dist_rtol relative of the exact ones.  Returns the number of
+    ambiguous ranks (reported, not failed).
+    """
+    X_test = np.asarray(X_test, np.float64)
+    X_train = np.asarray(X_train, np.float64)
+    if ref is None:
+        ref = oracle.exact_knn_f64(X_test, X_train, k, return_boundary=True)
+    rd, ri, rb = ref
+    dist = np.asarray(dist, np.float64)
+    idx = np.asarray(idx)
+    assert idx.shape == ri.shape and dist.shape == rd.shape
+    got_sq = dist if squared else dist ** 2
+    # distances: within dist_rtol, or (at a swapped near-tie) within the tie width
+    same = idx == ri
+    tol = np.where(same, dist_rtol * np.abs(rd), eps_rel * np.maximum(1.0, np.abs(rd)))
+    tol = np.maximum(tol, 4 * dist_rtol * np.abs(rd)) + 1e-12
+    err = np.abs(got_sq - rd)
+    assert np.all(err <= tol), f"distance error {np.max(err / np.maximum(np.abs(rd), 1e-30))}"
+    bad, ambiguous, _ = oracle.near_tie_report(X_test, X_train, idx, rd, ri, rb, eps_rel)
+    assert bad == 0, f"{bad} index mismatches outside near-ties"
+    return ambiguous

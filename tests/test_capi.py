@@ -1,0 +1,110 @@
+"""The C ABI library without a GPU: it loads, exports every symbol the header
+declares, sizes workspaces, and rejects bad arguments before touching the
+device.  The Python surface fails loudly when no GPU is present."""
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+from pdist_aggregation_amd import _capi
+import pdist_aggregation_amd as pa
+
+HEADER = pathlib.Path(__file__).resolve().parents[1] / "include" / "pdist_knn.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(knn_\w+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(_capi.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _capi.lib()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert lib.knn_abi_version() == 1
+
+
+def test_workspace_size():
+    lib = _capi.lib()
+    ws = lib.knn_argkmin_workspace_size
+    a = ws(4, 1_000_000, 10_000, 128, 10, 0, 0)
+    b = ws(4, 1_000_000, 200_000, 128, 10, 0, 0)
+    assert 0 < a < b
+    assert ws(8, 10_000, 1000, 50, 10, 0, 0) > 0
+    assert ws(4, 10, 3, 4, 11, 0, 0) == 0  # k > n_train
+    assert ws(4, 10, 3, 4, 0, 0, 0) == 0  # k == 0
+    assert ws(3, 10, 3, 4, 1, 0, 0) == 0  # bad dtype
+    assert ws(8, 100, 3, 4, 1, 1, 0) == 0  # f64 + bf16
+
+
+def _call(**over):
+    lib = _capi.lib()
+    args = dict(Xtr=1, ntr=100, Xte=1, nte=10, dim=8, k=5, chunk=256, strategy=0, prec=0,
+                ovs=0, flags=0, threads=1, dist=1, idx=1, ws=None, wsb=0, stream=None,
+                stats=None)
+    args.update(over)
+    return lib.knn_argkmin_f32(*args.values())
+
+
+@pytest.mark.parametrize("over,needle", [
+    ({"k": 0}, "k must be positive"),
+    ({"k": 101}, "k must not exceed n_train"),
+    ({"strategy": 7}, "strategy"),
+    ({"chunk": 0}, "chunk_size"),
+    ({"dim": 0}, "dim"),
+    ({"ovs": 3}, "oversample"),
+    ({"Xtr": None}, "null"),
+    ({"ws": 1, "wsb": 16}, "workspace"),
+])
+def test_bad_arguments_are_rejected_before_any_device_work(over, needle):
+    code = _call(**over)
+    assert code in (_capi.KNN_EINVAL, _capi.KNN_EWORKSPACE)
+    assert needle in _capi.lib().knn_last_error().decode()
+
+
+def test_f64_rejects_low_precision():
+    lib = _capi.lib()
+    code = lib.knn_argkmin_f64(1, 100, 1, 10, 8, 5, 256, 0, 1, 0, 0, 1, 1, 1, None, 0, None,
+                               None)
+    assert code == _capi.KNN_EINVAL
+
+
+def test_python_surface_rejects_bad_strategy_like_the_reference():
+    with pytest.raises(RuntimeError, match="strategy 'bogus' not supported"):
+        pa.parallel_knn(np.zeros((4, 2)), np.zeros((2, 2)), 1, strategy="bogus")
+
+
+def test_no_cpu_fallback():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(RuntimeError, match="needs a visible AMD GPU"):
+        pa.parallel_knn(np.random.rand(10, 3), np.random.rand(4, 3), 2)
+
+
+@pytest.mark.parametrize("strategy", ["auto", "chunk_on_train", "chunk_on_test"])
+def test_n_parallel_chunks_matches_port(strategy):
+    import oracle
+
+    rs = np.random.RandomState(0)
+    Y = rs.rand(777, 3)
+    X = rs.rand(1500, 3)
+    for chunk in (1, 20, 64, 256, 5000):
+        _, n = oracle.port_parallel_knn(Y, X, 2, chunk_size=chunk, strategy=strategy,
+                                        n_threads=2)
+        assert pa.n_parallel_chunks(strategy, 777, 1500, chunk, 2) == n
+
+
+def test_effective_n_threads_env(monkeypatch):
+    monkeypatch.setenv("OMP_NUM_THREADS", "3")
+    assert pa.effective_n_threads() == 3
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert 1 <= pa.effective_n_threads() <= (__import__("os").cpu_count() or 1)

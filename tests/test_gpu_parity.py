@@ -1,0 +1,214 @@
+"""Parity of the HIP path with the oracle, through the C ABI (needs a GPU).
+
+* the reference's CPU-runnable workload (1,000 x 10,000 x 50, float64, k=10)
+  and a subset of the reference test grid, against the committed sklearn
+  fixtures, for every strategy and several chunk sizes (the result must not
+  depend on them, as in tests/test_correctness.py of the reference);
+* float32 inputs against the exact FP64 oracle, near-ties exempt and counted;
+* edge cases: exact ties from duplicate rows, k = n_train, one query, d from 1
+  to 300, sizes that are not multiples of any tile;
+* the 10,000 x 1,000,000 x 128 float32 workload: a 1,000-query subsample
+  against the FP64 oracle and size-independent properties on every row.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import pdist_aggregation_amd as pa
+from conftest import golden_manifest, load_golden
+from knn_checks import EPS_F32, EPS_F64, assert_knn_parity, assert_sorted_unique
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", sorted(golden_manifest()))
+def test_f64_matches_sklearn_fixture(require_gpu, name):
+    X_train, X_test, k, sk_dist, sk_idx = load_golden(name)
+    _say(f"[f64 fixture] {name} train={X_train.shape} test={X_test.shape} k={k}")
+    ref = None
+    for strategy in ("auto", "chunk_on_train", "chunk_on_test"):
+        for chunk in (256, 4096):
+            (dist, idx), n_chunks = pa.parallel_knn(X_train, X_test, k, chunk_size=chunk,
+                                                    strategy=strategy, return_distance=True)
+            assert dist.dtype == np.float64 and idx.dtype == np.int64
+            np.testing.assert_almost_equal(dist, sk_dist)
+            if not np.array_equal(idx, sk_idx):
+                if ref is None:
+                    ref = oracle.exact_knn_f64(X_test, X_train, k, return_boundary=True)
+                assert_knn_parity(X_test, X_train, dist, idx, k, EPS_F64, squared=False,
+                                  ref=ref)
+            assert n_chunks == pa.n_parallel_chunks(strategy, X_train.shape[0],
+                                                    X_test.shape[0], chunk,
+                                                    pa.effective_n_threads())
+
+
+def test_workload_cpu_case_squared_f64(require_gpu):
+    """1,000 x 10,000 x 50 float64, k=10, chunk 256, squared distances."""
+    X_train, X_test, k, _, sk_idx = load_golden("workload_1000x10000x50_k10")
+    (dist, idx), _ = pa.parallel_knn(X_train, X_test, k, chunk_size=256, return_distance=True,
+                                     squared=True)
+    np.testing.assert_array_equal(idx, sk_idx)
+    ref_d, ref_i = oracle.exact_knn_f64(X_test, X_train, k)
+    np.testing.assert_allclose(dist, ref_d, rtol=1e-12)
+    # the C++ restatement of the reference engine agrees too
+    (pd, pi), _ = oracle.port_parallel_knn(X_train, X_test, k, chunk_size=256,
+                                           return_distance=True)
+    np.testing.assert_array_equal(idx, pi)
+    np.testing.assert_allclose(np.sqrt(dist), pd, rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", sorted(golden_manifest()))
+def test_f32_matches_oracle(require_gpu, name):
+    X_train, X_test, k, _, _ = load_golden(name)
+    Y = X_train.astype(np.float32)
+    X = X_test.astype(np.float32)
+    _say(f"[f32] {name}")
+    (dist, idx), _ = pa.parallel_knn(Y, X, k, return_distance=True, squared=True)
+    assert dist.dtype == np.float32
+    assert_sorted_unique(idx, dist, Y.shape[0])
+    amb = assert_knn_parity(X, Y, dist, idx, k, EPS_F32, squared=True)
+    _say(f"    ambiguous ranks: {amb}")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 2, 3, 7, 16, 50, 128, 129, 300])
+def test_dims_and_ragged_sizes(require_gpu, dtype, d):
+    rs = np.random.RandomState(d)
+    Y = rs.rand(1237, d).astype(dtype)
+    X = rs.rand(131, d).astype(dtype)
+    _say(f"[dims] d={d} {np.dtype(dtype).name}")
+    for k in (1, 5, 64):
+        (dist, idx), _ = pa.parallel_knn(Y, X, k, return_distance=True, squared=True)
+        assert_knn_parity(X, Y, dist, idx, k, EPS_F32 if dtype == np.float32 else EPS_F64)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_exact_ties_lower_index_first(require_gpu, dtype):
+    rs = np.random.RandomState(0)
+    Y = rs.rand(5000, 8).astype(dtype)
+    dup = np.arange(11, 5000, 37)
+    Y[dup] = Y[7]
+    X = rs.rand(70, 8).astype(dtype)
+    X[0] = Y[7]
+    X[1] = Y[7] + dtype(1e-3)
+    k = 40
+    (dist, idx), _ = pa.parallel_knn(Y, X, k, return_distance=True, squared=True)
+    expect = np.concatenate([[7], dup])[:k]
+    np.testing.assert_array_equal(idx[0], expect)
+    assert np.all(dist[0] == 0)
+    assert_sorted_unique(idx, dist, Y.shape[0])
+    assert_knn_parity(X, Y, dist, idx, k, 1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_k_equals_n_train_and_single_query(require_gpu, dtype):
+    rs = np.random.RandomState(5)
+    Y = rs.rand(300, 5).astype(dtype)
+    X = rs.rand(1, 5).astype(dtype)
+    (dist, idx), _ = pa.parallel_knn(Y, X, 300, return_distance=True, squared=True)
+    assert sorted(idx[0]) == list(range(300))
+    assert_knn_parity(X, Y, dist, idx, 300, 1e-6)
+    (dist, idx), _ = pa.parallel_knn(Y, X, 1, return_distance=True)
+    bd, bi = oracle.brute_force_knn(X, Y, 1)
+    assert idx[0, 0] == bi[0, 0]
+
+
+def test_large_k(require_gpu):
+    rs = np.random.RandomState(9)
+    Y = rs.rand(3000, 10)
+    X = rs.rand(257, 10)
+    (dist, idx), _ = pa.parallel_knn(Y, X, 1000, return_distance=True, squared=True)
+    assert_knn_parity(X, Y, dist, idx, 1000, EPS_F64)
+
+
+def test_device_tensors_in_and_out(require_gpu):
+    torch = require_gpu
+    rs = np.random.RandomState(2)
+    Y = rs.rand(4000, 33).astype(np.float32)
+    X = rs.rand(300, 33).astype(np.float32)
+    Yd = torch.from_numpy(Y).cuda()
+    Xd = torch.from_numpy(X).cuda()
+    (dist, idx), _ = pa.parallel_knn(Yd, Xd, 7, return_distance=True)
+    assert dist.is_cuda and idx.is_cuda and idx.dtype == torch.int64
+    (hd, hi), _ = pa.parallel_knn(Y, X, 7, return_distance=True)
+    np.testing.assert_array_equal(idx.cpu().numpy(), hi)
+    np.testing.assert_array_equal(dist.cpu().numpy(), hd)
+
+
+def test_estimators_and_sycl_surface(require_gpu):
+    from pdist_aggregation_amd import sycl_compat
+
+    rs = np.random.RandomState(4)
+    Y = rs.rand(2000, 6)
+    X = rs.rand(100, 6)
+    ref_d, ref_i = oracle.exact_knn_f64(X, Y, 4)
+    for cls in (pa.NearestNeighborsParallelAuto, pa.NearestNeighborsParallelXtrain,
+                pa.NearestNeighborsParallelXtest):
+        (d, i), n = cls(n_neighbors=4).fit(Y).kneighbors(X, return_distance=True)
+        np.testing.assert_array_equal(i, ref_i)
+        np.testing.assert_allclose(d, np.sqrt(ref_d), rtol=1e-12)
+        i2, n2 = cls(n_neighbors=4).fit(Y).kneighbors(X)
+        np.testing.assert_array_equal(i2, ref_i)
+        assert n2 == n
+    i, d = sycl_compat.NearestNeighbors(n_neighbors=4).fit(Y).kneighbors(X, return_distance=True)
+    np.testing.assert_array_equal(i, ref_i)
+    np.testing.assert_allclose(d, ref_d, rtol=1e-12)  # squared, as the SYCL backend
+    with pytest.raises(ValueError):
+        sycl_compat.parallel_knn(Y, X, 0)
+    with pytest.raises(ValueError):
+        sycl_compat.parallel_knn(Y, X, 3, strategy="chunk_on_train")
+
+
+def test_input_validation(require_gpu):
+    Y = np.random.rand(50, 4)
+    X = np.random.rand(5, 4)
+    with pytest.raises(ValueError):
+        pa.parallel_knn(Y, X, 51)
+    with pytest.raises(ValueError):
+        pa.parallel_knn(Y, np.random.rand(5, 3), 2)
+    with pytest.raises(TypeError):
+        pa.parallel_knn(Y, X.astype(np.float32), 2)
+    with pytest.raises(ValueError):
+        pa.parallel_knn(Y, np.asfortranarray(np.random.rand(5, 4)), 2)
+    with pytest.raises(RuntimeError):
+        pa.parallel_knn(Y, X, 2, strategy="chunk_on_nothing")
+
+
+def _bench_scale_inputs():
+    """10,000 queries and 1,000,000 train rows, d=128, U[0,1) float32,
+    default_rng(0), train drawn first."""
+    g = np.random.default_rng(0)
+    Y = g.random((1_000_000, 128), dtype=np.float32)
+    X = g.random((10_000, 128), dtype=np.float32)
+    return Y, X
+
+
+@pytest.mark.slow
+def test_full_scale_f32_subsample_and_properties(require_gpu):
+    torch = require_gpu
+    Y, X = _bench_scale_inputs()
+    _say("[full] 10000 x 1000000 x 128 f32, k=10")
+    Yd = torch.from_numpy(Y).cuda()
+    Xd = torch.from_numpy(X).cuda()
+    (dist, idx), _, st = pa.parallel_knn(Yd, Xd, 10, chunk_size=4096, return_distance=True,
+                                         squared=True, return_stats=True)
+    _say(f"    stats {st}")
+    dist = dist.cpu().numpy()
+    idx = idx.cpu().numpy()
+    # every row: sorted, unique, in range, distance = exact recomputation
+    assert_sorted_unique(idx, dist, Y.shape[0])
+    sub = np.arange(0, 10_000, 97)
+    diff = Y[idx[sub]].astype(np.float64) - X[sub][:, None, :].astype(np.float64)
+    np.testing.assert_allclose(dist[sub], (diff ** 2).sum(-1), rtol=1e-6)
+    # 1,000-query subsample against the FP64 oracle
+    q = np.arange(1000)
+    amb = assert_knn_parity(X[q], Y, dist[q], idx[q], 10, EPS_F32, squared=True)
+    _say(f"    ambiguous ranks in 1000 queries: {amb}; uncertified {st['n_uncertified']}")
+    assert st["n_uncertified"] == 0
