@@ -66,32 +66,52 @@ struct StageTimer {
 
 struct Plan {
   int64_t n_train = 0, n_test = 0, dim = 0, k = 0;
-  int kc = 0, cap = 0, n_splits = 1, path = 0;
+  int kc = 0, cap = 0, n_splits = 1, path = 0, dm = 0;
   int64_t rows_per_split = 0, n_qblocks = 0;
   size_t elem = 4;
   // workspace offsets
   size_t off_scalars = 0, off_ynorm = 0, off_tau = 0, off_cand_d = 0, off_cand_i = 0,
-         off_part_d = 0, off_part_i = 0, total = 0;
+         off_part_d = 0, off_part_i = 0, off_mu = 0, off_xq = 0, off_yq = 0, off_xnorm = 0,
+         off_xerr = 0, total = 0;
 };
 
+constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2;
+
+// Split count for a grid of n_qblocks query blocks: enough workgroups to keep
+// every CU busy several times over, each split at least min_steps row steps,
+// the merged per-query list within the merge kernel's LDS, and (MFMA) a
+// multiple of 8 so a split's query blocks share one XCD.
+int choose_splits(int64_t n_qblocks, int64_t n_train, int row_step, int kc, int64_t target,
+                  bool multiple_of_8) {
+  int64_t splits = std::max<int64_t>(1, (target + n_qblocks / 2) / n_qblocks);
+  splits = std::min<int64_t>(splits, std::max<int64_t>(1, n_train / (8 * row_step)));
+  splits = std::min<int64_t>(splits, std::max(1, knn::kMaxMerge / kc));
+  if (multiple_of_8 && splits >= 8) splits = splits / 8 * 8;
+  return (int)std::max<int64_t>(splits, 1);
+}
+
 int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim, int64_t k,
-              int precision, int64_t oversample) {
+              int precision, int64_t oversample, bool force_generic = false) {
   if (n_train <= 0 || n_test < 0 || dim <= 0)
     return fail(KNN_EINVAL, "n_train and dim must be positive and n_test non-negative");
   if (k <= 0) return fail(KNN_EINVAL, "k must be positive");
   if (k > n_train) return fail(KNN_EINVAL, "k must not exceed n_train");
   if (n_train > INT32_MAX) return fail(KNN_EUNSUPPORTED, "n_train must be below 2^31");
   if (dim > 65536) return fail(KNN_EUNSUPPORTED, "dim above 65536 is not supported");
-  if (precision != KNN_PRECISION_EXACT) {
-    if (elem != 4) return fail(KNN_EINVAL, "low precision search needs float32 inputs");
-    return fail(KNN_EUNSUPPORTED, "bf16/fp8 candidate search is not built in this library");
-  }
+  if (precision < KNN_PRECISION_EXACT || precision > KNN_PRECISION_FP8)
+    return fail(KNN_EINVAL, "unknown precision");
+  if (precision != KNN_PRECISION_EXACT && elem != 4)
+    return fail(KNN_EINVAL, "low precision search needs float32 inputs");
+  if (precision == KNN_PRECISION_FP8)
+    return fail(KNN_EUNSUPPORTED, "fp8 candidate search is not built in this library");
   int64_t kc;
   if (oversample > 0) {
     if (oversample < k) return fail(KNN_EINVAL, "oversample (k') must be >= k");
     kc = oversample;
-  } else {
+  } else if (precision == KNN_PRECISION_EXACT) {
     kc = k + std::max<int64_t>(8, k / 8);
+  } else {
+    kc = 4 * k;
   }
   kc = std::min<int64_t>(kc, n_train);
   if (kc > knn::kMaxCandidates) {
@@ -100,43 +120,89 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
                                          " is not supported");
     kc = knn::kMaxCandidates;
   }
+  p = Plan();
   p.n_train = n_train;
   p.n_test = n_test;
   p.dim = dim;
   p.k = k;
   p.elem = elem;
   p.kc = (int)kc;
-  p.path = 0;
-  p.cap = knn::next_pow2((int)kc + knn::kGenY);
-  p.n_qblocks = knn::ceil_div(std::max<int64_t>(n_test, 1), knn::kGenQ);
-  // Enough workgroups to cover 256 CUs several times, each split at least
-  // four row steps long, and the merged list within the merge kernel's LDS.
-  int64_t splits = knn::ceil_div(2048, p.n_qblocks);
-  splits = std::min<int64_t>(splits, knn::ceil_div(n_train, 4 * knn::kGenY));
-  splits = std::min<int64_t>(splits, knn::kMaxMerge / p.kc);
-  splits = std::max<int64_t>(splits, 1);
-  p.rows_per_split = knn::ceil_div(knn::ceil_div(n_train, splits), knn::kGenY) * knn::kGenY;
-  p.n_splits = (int)knn::ceil_div(n_train, p.rows_per_split);
 
+  const int mfma_cap = knn::next_pow2((int)kc + knn::kMfmaRowStep);
+  const bool mfma_cap_ok = mfma_cap <= knn::kMfmaMaxCap;
+  p.path = kPathGeneric;
+  if (precision == KNN_PRECISION_BF16) {
+    if (dim > 256 || !mfma_cap_ok)
+      return fail(KNN_EUNSUPPORTED, "bf16 search supports dim <= 256 and k' <= " +
+                                        std::to_string(knn::kMfmaMaxCap - knn::kMfmaRowStep));
+    p.path = kPathMfmaBF16;
+    p.dm = std::max(32, knn::next_pow2((int)dim));
+  } else if (!force_generic && elem == 4 && dim % 4 == 0 && dim >= 16 && dim <= 128 &&
+             mfma_cap_ok) {
+    p.path = kPathMfmaF32;
+    p.dm = std::max(32, knn::next_pow2((int)dim));
+  }
+
+  int bq, step;
+  if (p.path == kPathGeneric) {
+    bq = knn::kGenQ;
+    step = knn::kGenY;
+    p.cap = knn::next_pow2((int)kc + knn::kGenY);
+  } else {
+    bq = knn::kMfmaQueriesPerBlock;
+    step = knn::kMfmaRowStep;
+    p.cap = mfma_cap;
+  }
+  p.n_qblocks = knn::ceil_div(std::max<int64_t>(n_test, 1), bq);
+  const int splits = choose_splits(p.n_qblocks, n_train, step, p.kc,
+                                   p.path == kPathGeneric ? 2048 : 1024,
+                                   p.path != kPathGeneric);
+  p.rows_per_split = knn::ceil_div(knn::ceil_div(n_train, splits), step) * step;
+  p.n_splits = (int)knn::ceil_div(n_train, p.rows_per_split);
+  // An evenly divisible split count keeps the XCD-aware block order.
+  if (p.path != kPathGeneric && p.n_splits != splits) p.n_splits = splits;
+
+  const size_t celem = p.path == kPathGeneric ? elem : sizeof(float);
+  const int64_t nq1 = std::max<int64_t>(n_test, 1);
   size_t o = 0;
-  p.off_scalars = o;
-  o = align_up(o + sizeof(knn::DeviceScalars));
-  p.off_ynorm = o;
-  o = align_up(o + (size_t)n_train * elem);
-  p.off_tau = o;
-  o = align_up(o + (size_t)std::max<int64_t>(n_test, 1) * sizeof(unsigned long long));
-  const size_t ncand = (size_t)p.n_qblocks * p.n_splits * knn::kGenQ * p.cap;
-  p.off_cand_d = o;
-  o = align_up(o + ncand * elem);
-  p.off_cand_i = o;
-  o = align_up(o + ncand * sizeof(int));
-  const size_t npart = (size_t)std::max<int64_t>(n_test, 1) * p.n_splits * p.kc;
-  p.off_part_d = o;
-  o = align_up(o + npart * elem);
-  p.off_part_i = o;
-  o = align_up(o + npart * sizeof(int));
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = align_up(o + bytes);
+    return at;
+  };
+  p.off_scalars = take(sizeof(knn::DeviceScalars));
+  p.off_ynorm = take((size_t)n_train * celem);
+  p.off_tau = take((size_t)nq1 * sizeof(unsigned long long));
+  const size_t ncand = (size_t)p.n_qblocks * p.n_splits * bq * p.cap;
+  p.off_cand_d = take(ncand * celem);
+  p.off_cand_i = take(ncand * sizeof(int));
+  const size_t npart = (size_t)nq1 * p.n_splits * p.kc;
+  p.off_part_d = take(npart * celem);
+  p.off_part_i = take(npart * sizeof(int));
+  if (p.path == kPathMfmaBF16) {
+    p.off_mu = take(2 * (size_t)p.dm * sizeof(float));
+    p.off_xq = take((size_t)nq1 * p.dm * 2);
+    p.off_yq = take((size_t)n_train * p.dm * 2);
+    p.off_xnorm = take((size_t)nq1 * sizeof(float));
+    p.off_xerr = take((size_t)nq1 * sizeof(float));
+  }
   p.total = o;
   return KNN_OK;
+}
+
+// Workspace that covers the chosen plan and the generic fallback it may take
+// at run time (misaligned float32 pointers).
+size_t plan_workspace(size_t elem, int64_t n_train, int64_t n_test, int64_t dim, int64_t k,
+                      int precision, int64_t oversample) {
+  Plan p;
+  if (make_plan(p, elem, n_train, n_test, dim, k, precision, oversample) != KNN_OK) return 0;
+  size_t total = p.total;
+  if (p.path == kPathMfmaF32) {
+    Plan g;
+    if (make_plan(g, elem, n_train, n_test, dim, k, precision, oversample, true) == KNN_OK)
+      total = std::max(total, g.total);
+  }
+  return total;
 }
 
 template <class T>
@@ -153,6 +219,11 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   Plan p;
   int rc = make_plan(p, sizeof(T), n_train, n_test, dim, k, precision, oversample);
   if (rc != KNN_OK) return rc;
+  if (p.path == kPathMfmaF32 && ((reinterpret_cast<uintptr_t>(X_train) |
+                                  reinterpret_cast<uintptr_t>(X_test)) & 15) != 0) {
+    rc = make_plan(p, sizeof(T), n_train, n_test, dim, k, precision, oversample, true);
+    if (rc != KNN_OK) return rc;
+  }
 
   int threads = n_threads_hint > 0 ? n_threads_hint : (int)std::thread::hardware_concurrency();
   threads = std::max(threads, 1);
@@ -197,45 +268,105 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
                                sizeof(knn::DeviceScalars) / 8,
                                0ull, stream);
   knn::launch_fill_keys<float>(tau, n_test, knn::ordered_key(__builtin_huge_val()), stream);
-  // 2. norms
-  knn::launch_row_norms<T>(X_train, n_train, (int)dim, ynorm, sc, stream);
-  // 3. search
-  knn::SearchArgs<T> sa;
-  sa.X = X_test;
-  sa.Y = X_train;
-  sa.ynorm = ynorm;
-  sa.n_q = n_test;
-  sa.n_t = n_train;
-  sa.d = (int)dim;
-  sa.kc = p.kc;
-  sa.cap = p.cap;
-  sa.n_splits = p.n_splits;
-  sa.rows_per_split = p.rows_per_split;
-  sa.cand_d = reinterpret_cast<T*>(ws + p.off_cand_d);
-  sa.cand_i = reinterpret_cast<int*>(ws + p.off_cand_i);
-  sa.tau = tau;
-  sa.part_d = reinterpret_cast<T*>(ws + p.off_part_d);
-  sa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
-  timer.mark(1);
-  knn::launch_generic_search<T>(sa, stream);
-  timer.mark(2);
-  // 4. merge + exact re-rank
   knn::MergeArgs<T> ma;
+  ma.xnorm = nullptr;
+  ma.xerr = nullptr;
+  ma.unit_roundoff = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
+  ma.acc_terms = 2 * (int)dim + 4;
+  if (p.path == kPathGeneric) {
+    // 2. norms  3. search
+    knn::launch_row_norms<T>(X_train, n_train, (int)dim, ynorm, sc, stream);
+    knn::SearchArgs<T> sa;
+    sa.X = X_test;
+    sa.Y = X_train;
+    sa.ynorm = ynorm;
+    sa.n_q = n_test;
+    sa.n_t = n_train;
+    sa.d = (int)dim;
+    sa.kc = p.kc;
+    sa.cap = p.cap;
+    sa.n_splits = p.n_splits;
+    sa.rows_per_split = p.rows_per_split;
+    sa.cand_d = reinterpret_cast<T*>(ws + p.off_cand_d);
+    sa.cand_i = reinterpret_cast<int*>(ws + p.off_cand_i);
+    sa.tau = tau;
+    sa.part_d = reinterpret_cast<T*>(ws + p.off_part_d);
+    sa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
+    timer.mark(1);
+    knn::launch_generic_search<T>(sa, stream);
+    timer.mark(2);
+    ma.part_d = sa.part_d;
+    ma.part_i = sa.part_i;
+  } else if constexpr (sizeof(T) == 4) {
+    knn::MfmaArgs fa;
+    fa.dm = p.dm;
+    fa.n_q = n_test;
+    fa.n_t = n_train;
+    fa.kc = p.kc;
+    fa.cap = p.cap;
+    fa.n_qblocks = (int)p.n_qblocks;
+    fa.n_splits = p.n_splits;
+    fa.rows_per_split = p.rows_per_split;
+    fa.cand_d = reinterpret_cast<float*>(ws + p.off_cand_d);
+    fa.cand_i = reinterpret_cast<int*>(ws + p.off_cand_i);
+    fa.tau = tau;
+    fa.part_d = reinterpret_cast<float*>(ws + p.off_part_d);
+    fa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
+    fa.ynorm = reinterpret_cast<float*>(ynorm);
+    if (p.path == kPathMfmaF32) {
+      // 2. norms
+      knn::launch_row_norms<float>(reinterpret_cast<const float*>(X_train), n_train, (int)dim,
+                                   reinterpret_cast<float*>(ynorm), sc, stream);
+      fa.fmt = knn::kFmtF32;
+      fa.X = X_test;
+      fa.Y = X_train;
+      fa.ldx = dim;
+      fa.ldy = dim;
+      fa.d = (int)dim;
+    } else {
+      // 2. centre + bf16 rounding of both sets, norms and error bounds
+      float* mu = reinterpret_cast<float*>(ws + p.off_mu);
+      float* xnorm = reinterpret_cast<float*>(ws + p.off_xnorm);
+      float* xerr = reinterpret_cast<float*>(ws + p.off_xerr);
+      const float* Ytr = reinterpret_cast<const float*>(X_train);
+      const float* Xte = reinterpret_cast<const float*>(X_test);
+      knn::launch_column_mean(Ytr, std::min<int64_t>(n_train, 65536), (int)dim, mu + p.dm, mu,
+                              stream);
+      knn::launch_quantize_bf16(Ytr, n_train, (int)dim, p.dm, mu, ws + p.off_yq,
+                                reinterpret_cast<float*>(ynorm), nullptr, sc, true, stream);
+      knn::launch_quantize_bf16(Xte, n_test, (int)dim, p.dm, mu, ws + p.off_xq, xnorm, xerr, sc,
+                                false, stream);
+      fa.fmt = knn::kFmtBF16;
+      fa.X = ws + p.off_xq;
+      fa.Y = ws + p.off_yq;
+      fa.ldx = p.dm;
+      fa.ldy = p.dm;
+      fa.d = p.dm;
+      ma.xnorm = xnorm;
+      ma.xerr = xerr;
+      ma.unit_roundoff = 0x1p-24;
+      ma.acc_terms = 2 * p.dm + 4;
+    }
+    // 3. search
+    timer.mark(1);
+    if (!knn::launch_mfma_search(fa, stream)) {
+      if (owned) (void)hipFreeAsync(ws, stream);
+      return fail(KNN_EUNSUPPORTED, "no MFMA kernel for this feature count");
+    }
+    timer.mark(2);
+    ma.part_d = reinterpret_cast<const T*>(fa.part_d);
+    ma.part_i = fa.part_i;
+  }
+  // 4. merge + exact re-rank
   ma.X = X_test;
   ma.Y = X_train;
   ma.n_q = n_test;
   ma.n_t = n_train;
   ma.d = (int)dim;
-  ma.part_d = sa.part_d;
-  ma.part_i = sa.part_i;
   ma.n_splits = p.n_splits;
   ma.kc = p.kc;
   ma.k = (int)k;
   ma.squared = (flags & KNN_FLAG_SQUARED) ? 1 : 0;
-  ma.xnorm = nullptr;
-  ma.xerr = nullptr;
-  ma.unit_roundoff = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
-  ma.acc_terms = 2 * (int)dim + 4;
   ma.sc = sc;
   ma.dist_out = dist_out;
   ma.idx_out = idx_out;
@@ -298,10 +429,7 @@ int knn_argkmin_f64(const double* X_train, int64_t n_train, const double* X_test
 size_t knn_argkmin_workspace_size(int dtype_bytes, int64_t n_train, int64_t n_test, int64_t dim,
                                   int64_t k, int precision, int64_t oversample) {
   if (dtype_bytes != 4 && dtype_bytes != 8) return 0;
-  Plan p;
-  if (make_plan(p, (size_t)dtype_bytes, n_train, n_test, dim, k, precision, oversample) != KNN_OK)
-    return 0;
-  return p.total;
+  return plan_workspace((size_t)dtype_bytes, n_train, n_test, dim, k, precision, oversample);
 }
 
 const char* knn_last_error(void) { return g_last_error.c_str(); }

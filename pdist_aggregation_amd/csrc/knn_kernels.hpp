@@ -46,6 +46,50 @@ struct SearchArgs {
   int* part_i;
 };
 
+// ---------------------------------------------------------------------------
+// MFMA search (knn_mfma.hip)
+// ---------------------------------------------------------------------------
+constexpr int kFmtF32 = 0;   // v_mfma_f32_32x32x2_f32 on the caller's rows
+constexpr int kFmtBF16 = 1;  // v_mfma_f32_32x32x16_bf16 on quantised rows
+constexpr int kMfmaQueriesPerBlock = 256;  // 4 waves x 2 groups x 32 queries
+constexpr int kMfmaRowStep = 32;
+constexpr int kMfmaMaxCap = 512;  // per-query list capacity (wave sort in LDS)
+
+struct MfmaArgs {
+  int fmt;
+  int dm;                // padded feature count the kernel is compiled for
+  const void* X;         // queries (float or bf16), row stride ldx elements
+  const void* Y;         // train rows (float or bf16), row stride ldy elements
+  const float* ynorm;    // |y|^2 as used by the search
+  int64_t ldx, ldy;
+  int64_t n_q, n_t;
+  int d;                 // valid features (<= dm)
+  int kc, cap;
+  int n_qblocks, n_splits;
+  int64_t rows_per_split;  // multiple of kMfmaRowStep
+  float* cand_d;
+  int* cand_i;
+  unsigned long long* tau;
+  float* part_d;
+  int* part_i;
+};
+
+// Returns false when no instantiation matches (fmt, dm).
+bool launch_mfma_search(const MfmaArgs& a, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Low-precision preparation (knn_quant.hip)
+// ---------------------------------------------------------------------------
+// mu[f] = mean of feature f over the first n_sample rows of Y (f < d).
+void launch_column_mean(const float* Y, int64_t n_sample, int d, float* mu_sum, float* mu,
+                        hipStream_t s);
+// dst[r] = bf16(src[r] - mu) zero-padded to dm; norm[r] = |dst[r]|^2 (fp32);
+// err[r] >= |src[r] - mu - dst[r]| (nullable).  With track_max the maxima of
+// norm and err go to sc (max_ynorm_key, max_yerr_key).
+void launch_quantize_bf16(const float* src, int64_t n, int d, int dm, const float* mu,
+                          void* dst, float* norm, float* err, DeviceScalars* sc, bool track_max,
+                          hipStream_t s);
+
 template <class T>
 void launch_fill_keys(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 

@@ -14,16 +14,22 @@ EPS_F64 = 1e-10
 DIST_RTOL = 1e-5
 
 
-def assert_sorted_unique(idx, dist, n_train):
+def assert_sorted_unique(idx, dist, n_train, X=None, Y=None):
+    """Rows ascending, indices unique and in range.  With the inputs given,
+    equal exact FP64 distances must list the lower index first (float32
+    outputs can round two different distances to one value, so the tie rule
+    is checked on recomputed exact distances)."""
     idx = np.asarray(idx)
     dist = np.asarray(dist)
     assert idx.min() >= 0 and idx.max() < n_train
     assert np.all(np.diff(dist, axis=1) >= 0), "rows not sorted ascending"
     s = np.sort(idx, axis=1)
     assert np.all(s[:, 1:] != s[:, :-1]), "duplicate index in a row"
-    # equal distances list the lower index first
-    eq = np.diff(dist, axis=1) == 0
-    assert np.all(np.diff(idx, axis=1)[eq] > 0), "tie not broken by lower index"
+    if X is not None:
+        exact = oracle._direct_sqdist(np.asarray(X, np.float64), np.asarray(Y, np.float64), idx)
+        assert np.all(np.diff(exact, axis=1) >= 0), "rows not sorted by exact distance"
+        eq = np.diff(exact, axis=1) == 0
+        assert np.all(np.diff(idx, axis=1)[eq] > 0), "tie not broken by lower index"
 
 
 def assert_knn_parity(X_test, X_train, dist, idx, k, eps_rel, squared=True, dist_rtol=DIST_RTOL,

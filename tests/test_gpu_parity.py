@@ -72,7 +72,7 @@ def test_f32_matches_oracle(require_gpu, name):
     _say(f"[f32] {name}")
     (dist, idx), _ = pa.parallel_knn(Y, X, k, return_distance=True, squared=True)
     assert dist.dtype == np.float32
-    assert_sorted_unique(idx, dist, Y.shape[0])
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
     amb = assert_knn_parity(X, Y, dist, idx, k, EPS_F32, squared=True)
     _say(f"    ambiguous ranks: {amb}")
 
@@ -103,7 +103,7 @@ def test_exact_ties_lower_index_first(require_gpu, dtype):
     expect = np.concatenate([[7], dup])[:k]
     np.testing.assert_array_equal(idx[0], expect)
     assert np.all(dist[0] == 0)
-    assert_sorted_unique(idx, dist, Y.shape[0])
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
     assert_knn_parity(X, Y, dist, idx, k, 1e-12)
 
 
@@ -212,3 +212,72 @@ def test_full_scale_f32_subsample_and_properties(require_gpu):
     amb = assert_knn_parity(X[q], Y, dist[q], idx[q], 10, EPS_F32, squared=True)
     _say(f"    ambiguous ranks in 1000 queries: {amb}; uncertified {st['n_uncertified']}")
     assert st["n_uncertified"] == 0
+
+
+@pytest.mark.parametrize("d,expect_path", [(16, 1), (32, 1), (48, 1), (64, 1), (100, 1),
+                                           (128, 1), (50, 0), (132, 0)])
+def test_f32_kernel_path_selection(require_gpu, d, expect_path):
+    """Aligned float32 shapes with 16 <= d <= 128 run on the MFMA kernel."""
+    rs = np.random.RandomState(d)
+    Y = rs.rand(2111, d).astype(np.float32)
+    X = rs.rand(301, d).astype(np.float32)
+    (dist, idx), _, st = pa.parallel_knn(Y, X, 9, return_distance=True, squared=True,
+                                         return_stats=True)
+    assert st["kernel_path"] == expect_path
+    assert_knn_parity(X, Y, dist, idx, 9, EPS_F32)
+    assert st["n_uncertified"] == 0
+
+
+@pytest.mark.parametrize("k", [1, 10, 100, 400])
+def test_mfma_f32_many_k(require_gpu, k):
+    rs = np.random.RandomState(k)
+    Y = rs.rand(20000, 64).astype(np.float32)
+    X = rs.rand(513, 64).astype(np.float32)
+    (dist, idx), _, st = pa.parallel_knn(Y, X, k, return_distance=True, squared=True,
+                                         return_stats=True)
+    assert st["kernel_path"] == 1
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, k, EPS_F32)
+
+
+def _gaussian_mixture(rs, n, d, centres, sigma):
+    lab = rs.randint(0, centres.shape[0], n)
+    return (centres[lab] + sigma * rs.randn(n, d)).astype(np.float32)
+
+
+@pytest.mark.parametrize("d", [32, 50, 128, 200])
+def test_bf16_rerank_exact_when_certified(require_gpu, d):
+    """BF16 candidates + exact re-rank: certified rows equal the FP64 answer;
+    recall@k over all rows >= 0.999."""
+    rs = np.random.RandomState(100 + d)
+    Y = rs.rand(30000, d).astype(np.float32)
+    X = rs.rand(400, d).astype(np.float32)
+    k = 10
+    (dist, idx), _, st = pa.parallel_knn(Y, X, k, return_distance=True, squared=True,
+                                         precision="bf16", return_stats=True)
+    assert st["kernel_path"] == 2 and st["k_candidates"] == 4 * k
+    ref = oracle.exact_knn_f64(X, Y, k, return_boundary=True)
+    rec = oracle.recall_at_k(idx, ref[1])
+    _say(f"    bf16 d={d}: recall@{k}={rec:.5f} uncertified={st['n_uncertified']}")
+    assert rec >= 0.999
+    if st["n_uncertified"] == 0:
+        assert_knn_parity(X, Y, dist, idx, k, EPS_F32)
+
+
+def test_bf16_gaussian_mixture_k32(require_gpu):
+    """Mixture of Gaussians (centres U[-1,1]^128, sigma 0.1), k=32, k'=128."""
+    rs = np.random.RandomState(7)
+    centres = rs.uniform(-1, 1, (100, 128))
+    Y = _gaussian_mixture(rs, 50000, 128, centres, 0.1)
+    X = _gaussian_mixture(rs, 300, 128, centres, 0.1)
+    (dist, idx), _, st = pa.parallel_knn(Y, X, 32, return_distance=True, squared=True,
+                                         precision="bf16", oversample=128, return_stats=True)
+    ref = oracle.exact_knn_f64(X, Y, 32, return_boundary=True)
+    rec = oracle.recall_at_k(idx, ref[1])
+    err = np.max(np.abs(dist.astype(np.float64) - ref[0]) / np.maximum(ref[0], 1e-30))
+    _say(f"    bf16 mixture: recall@32={rec:.5f} max rel dist err={err:.2e} "
+         f"uncertified={st['n_uncertified']}")
+    assert rec >= 0.999
+    # returned distances are exact FP64 recomputations of the returned rows
+    got = oracle._direct_sqdist(X, Y, idx)
+    np.testing.assert_allclose(dist, got, rtol=1e-6)
