@@ -72,10 +72,10 @@ struct Plan {
   // workspace offsets
   size_t off_scalars = 0, off_ynorm = 0, off_tau = 0, off_cand_d = 0, off_cand_i = 0,
          off_part_d = 0, off_part_i = 0, off_mu = 0, off_xq = 0, off_yq = 0, off_xnorm = 0,
-         off_xerr = 0, total = 0;
+         off_xerr = 0, off_xscale = 0, off_yscale = 0, total = 0;
 };
 
-constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2;
+constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3;
 
 // Split count for a grid of n_qblocks query blocks: enough workgroups to keep
 // every CU busy several times over, each split at least min_steps row steps,
@@ -102,8 +102,6 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     return fail(KNN_EINVAL, "unknown precision");
   if (precision != KNN_PRECISION_EXACT && elem != 4)
     return fail(KNN_EINVAL, "low precision search needs float32 inputs");
-  if (precision == KNN_PRECISION_FP8)
-    return fail(KNN_EUNSUPPORTED, "fp8 candidate search is not built in this library");
   int64_t kc;
   if (oversample > 0) {
     if (oversample < k) return fail(KNN_EINVAL, "oversample (k') must be >= k");
@@ -131,12 +129,12 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   const int mfma_cap = knn::next_pow2((int)kc + knn::kMfmaRowStep);
   const bool mfma_cap_ok = mfma_cap <= knn::kMfmaMaxCap;
   p.path = kPathGeneric;
-  if (precision == KNN_PRECISION_BF16) {
+  if (precision != KNN_PRECISION_EXACT) {
     if (dim > 256 || !mfma_cap_ok)
-      return fail(KNN_EUNSUPPORTED, "bf16 search supports dim <= 256 and k' <= " +
+      return fail(KNN_EUNSUPPORTED, "bf16/fp8 search supports dim <= 256 and k' <= " +
                                         std::to_string(knn::kMfmaMaxCap - knn::kMfmaRowStep));
-    p.path = kPathMfmaBF16;
-    p.dm = std::max(32, knn::next_pow2((int)dim));
+    p.path = precision == KNN_PRECISION_BF16 ? kPathMfmaBF16 : kPathMfmaFP8;
+    p.dm = std::max(p.path == kPathMfmaBF16 ? 32 : 64, knn::next_pow2((int)dim));
   } else if (!force_generic && elem == 4 && dim % 4 == 0 && dim >= 16 && dim <= 128 &&
              mfma_cap_ok) {
     p.path = kPathMfmaF32;
@@ -179,12 +177,17 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   const size_t npart = (size_t)nq1 * p.n_splits * p.kc;
   p.off_part_d = take(npart * celem);
   p.off_part_i = take(npart * sizeof(int));
-  if (p.path == kPathMfmaBF16) {
+  if (p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8) {
+    const size_t qe = p.path == kPathMfmaBF16 ? 2 : 1;
     p.off_mu = take(2 * (size_t)p.dm * sizeof(float));
-    p.off_xq = take((size_t)nq1 * p.dm * 2);
-    p.off_yq = take((size_t)n_train * p.dm * 2);
+    p.off_xq = take((size_t)nq1 * p.dm * qe);
+    p.off_yq = take((size_t)n_train * p.dm * qe);
     p.off_xnorm = take((size_t)nq1 * sizeof(float));
     p.off_xerr = take((size_t)nq1 * sizeof(float));
+    if (p.path == kPathMfmaFP8) {
+      p.off_xscale = take((size_t)nq1);
+      p.off_yscale = take((size_t)n_train);
+    }
   }
   p.total = o;
   return KNN_OK;
@@ -299,6 +302,8 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     ma.part_i = sa.part_i;
   } else if constexpr (sizeof(T) == 4) {
     knn::MfmaArgs fa;
+    fa.xscale = nullptr;
+    fa.yscale = nullptr;
     fa.dm = p.dm;
     fa.n_q = n_test;
     fa.n_t = n_train;
@@ -332,11 +337,23 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       const float* Xte = reinterpret_cast<const float*>(X_test);
       knn::launch_column_mean(Ytr, std::min<int64_t>(n_train, 65536), (int)dim, mu + p.dm, mu,
                               stream);
-      knn::launch_quantize_bf16(Ytr, n_train, (int)dim, p.dm, mu, ws + p.off_yq,
-                                reinterpret_cast<float*>(ynorm), nullptr, sc, true, stream);
-      knn::launch_quantize_bf16(Xte, n_test, (int)dim, p.dm, mu, ws + p.off_xq, xnorm, xerr, sc,
-                                false, stream);
-      fa.fmt = knn::kFmtBF16;
+      if (p.path == kPathMfmaBF16) {
+        knn::launch_quantize_bf16(Ytr, n_train, (int)dim, p.dm, mu, ws + p.off_yq,
+                                  reinterpret_cast<float*>(ynorm), nullptr, sc, true, stream);
+        knn::launch_quantize_bf16(Xte, n_test, (int)dim, p.dm, mu, ws + p.off_xq, xnorm, xerr,
+                                  sc, false, stream);
+        fa.fmt = knn::kFmtBF16;
+      } else {
+        auto* xs = reinterpret_cast<unsigned char*>(ws + p.off_xscale);
+        auto* ys = reinterpret_cast<unsigned char*>(ws + p.off_yscale);
+        knn::launch_quantize_fp8(Ytr, n_train, (int)dim, p.dm, mu, ws + p.off_yq, ys,
+                                 reinterpret_cast<float*>(ynorm), nullptr, sc, true, stream);
+        knn::launch_quantize_fp8(Xte, n_test, (int)dim, p.dm, mu, ws + p.off_xq, xs, xnorm, xerr,
+                                 sc, false, stream);
+        fa.fmt = knn::kFmtFP8;
+        fa.xscale = xs;
+        fa.yscale = ys;
+      }
       fa.X = ws + p.off_xq;
       fa.Y = ws + p.off_yq;
       fa.ldx = p.dm;

@@ -51,6 +51,7 @@ struct SearchArgs {
 // ---------------------------------------------------------------------------
 constexpr int kFmtF32 = 0;   // v_mfma_f32_32x32x2_f32 on the caller's rows
 constexpr int kFmtBF16 = 1;  // v_mfma_f32_32x32x16_bf16 on quantised rows
+constexpr int kFmtFP8 = 2;   // v_mfma_scale_f32_32x32x64_f8f6f4, E4M3 + row scales
 constexpr int kMfmaQueriesPerBlock = 256;  // 4 waves x 2 groups x 32 queries
 constexpr int kMfmaRowStep = 32;
 constexpr int kMfmaMaxCap = 512;  // per-query list capacity (wave sort in LDS)
@@ -61,6 +62,8 @@ struct MfmaArgs {
   const void* X;         // queries (float or bf16), row stride ldx elements
   const void* Y;         // train rows (float or bf16), row stride ldy elements
   const float* ynorm;    // |y|^2 as used by the search
+  const unsigned char* xscale;  // FP8: E8M0 row scales of X and Y
+  const unsigned char* yscale;
   int64_t ldx, ldy;
   int64_t n_q, n_t;
   int d;                 // valid features (<= dm)
@@ -89,6 +92,11 @@ void launch_column_mean(const float* Y, int64_t n_sample, int d, float* mu_sum, 
 void launch_quantize_bf16(const float* src, int64_t n, int d, int dm, const float* mu,
                           void* dst, float* norm, float* err, DeviceScalars* sc, bool track_max,
                           hipStream_t s);
+
+// OCP E4M3 with a power-of-two scale per row (E8M0 byte in scale[r]).
+void launch_quantize_fp8(const float* src, int64_t n, int d, int dm, const float* mu, void* dst,
+                         unsigned char* scale, float* norm, float* err, DeviceScalars* sc,
+                         bool track_max, hipStream_t s);
 
 template <class T>
 void launch_fill_keys(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);

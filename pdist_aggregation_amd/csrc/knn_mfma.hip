@@ -25,7 +25,9 @@
 //
 // Formats: F32 = v_mfma_f32_32x32x2_f32 on the caller's float32 rows;
 //          BF16 = v_mfma_f32_32x32x16_bf16 on rows pre-quantised by
-//          knn_quant.hip (centred, zero-padded to DM).
+//          knn_quant.hip (centred, zero-padded to DM);
+//          FP8 = v_mfma_scale_f32_32x32x64_f8f6f4 (OCP E4M3 operands) with the
+//          per-row power-of-two scales applied by the instruction itself.
 #include "knn_common.hpp"
 #include "knn_kernels.hpp"
 
@@ -37,6 +39,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBY = 32;  // train rows per step (one MFMA row block)
 
@@ -59,6 +62,12 @@ struct Fmt<kFmtBF16> {
   static constexpr int esize = 2;
   static constexpr int KL = 8;
   using breg = bf16x8;
+};
+template <>
+struct Fmt<kFmtFP8> {
+  static constexpr int esize = 1;
+  static constexpr int KL = 32;
+  using breg = i32x8;
 };
 
 // LDS row pitch in bytes: DM elements + 16 bytes, so the 16-byte reads of 8
@@ -117,6 +126,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
 
   __shared__ __align__(16) unsigned char ytile[2][kBY * PITCH];
   __shared__ __align__(16) float yn_s[2][kBY];
+  __shared__ int ye_s[2][kBY];  // FP8: E8M0 row scales
   __shared__ int cnt_s[BQ];
   __shared__ float tau_s[BQ];
   extern __shared__ __align__(16) unsigned char dyn_smem[];  // 4 waves x cap x 8 B
@@ -151,6 +161,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
 
   // ---- resident queries: B operand, scaled by -2 ------------------------------
   typename F::breg xr[QG][NSTEP];
+  int xsc[QG];  // FP8: E8M0 scale of -2 x (the row scale times 2)
   float tau[QG];
 #pragma unroll
   for (int g = 0; g < QG; ++g) {
@@ -169,7 +180,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
         xr[g][s + 2] = -2.f * v[2];
         xr[g][s + 3] = -2.f * v[3];
       }
-    } else {
+    } else if constexpr (FMT == kFmtBF16) {
       const __bf16* xrow = reinterpret_cast<const __bf16*>(a.X) + q * a.ldx;
 #pragma unroll
       for (int s = 0; s < NSTEP; ++s) {
@@ -179,6 +190,24 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
         for (int e = 0; e < 8; ++e) v[e] = (__bf16)(-2.f * (float)v[e]);
         xr[g][s] = v;
       }
+    } else {
+      // -2 x: flip every sign bit, and double the power-of-two row scale.
+      const unsigned char* xrow = reinterpret_cast<const unsigned char*>(a.X) + q * a.ldx;
+#pragma unroll
+      for (int s = 0; s < NSTEP; ++s) {
+        i32x8 v = {};
+        if (qok) {
+          const i32x4 lo = *reinterpret_cast<const i32x4*>(xrow + h * (DM / 2) + s * 32);
+          const i32x4 hi = *reinterpret_cast<const i32x4*>(xrow + h * (DM / 2) + s * 32 + 16);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            v[e] = lo[e] ^ (int)0x80808080u;
+            v[e + 4] = hi[e] ^ (int)0x80808080u;
+          }
+        }
+        xr[g][s] = v;
+      }
+      xsc[g] = qok ? (int)a.xscale[q] + 1 : 127;
     }
   }
   for (int i = tid; i < BQ; i += kThreads) {
@@ -189,6 +218,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   // ---- tile loader: global -> registers -> LDS -------------------------------
   i32x4 pre[CPT];
   float pre_n = 0.f;
+  int pre_e = 127;
   auto load_tile = [&](int64_t y0) {
 #pragma unroll
     for (int i = 0; i < CPT; ++i) {
@@ -206,6 +236,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     if (tid < kBY) {
       const int64_t y = y0 + tid;
       pre_n = y < yend ? a.ynorm[y] : __builtin_huge_valf();
+      if constexpr (FMT == kFmtFP8) pre_e = y < yend ? (int)a.yscale[y] : 127;
     }
   };
   auto store_tile = [&](int buf) {
@@ -216,7 +247,10 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
       const int col = ch % (DM * F::esize / 16);
       if (ch < CHUNKS) *reinterpret_cast<i32x4*>(&ytile[buf][row * PITCH + col * 16]) = pre[i];
     }
-    if (tid < kBY) yn_s[buf][tid] = pre_n;
+    if (tid < kBY) {
+      yn_s[buf][tid] = pre_n;
+      ye_s[buf][tid] = pre_e;
+    }
   };
 
   load_tile(ybeg);
@@ -254,13 +288,27 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
           for (int g = 0; g < QG; ++g)
             acc[g] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], xr[g][s + e], acc[g], 0, 0, 0);
       }
-    } else {
+    } else if constexpr (FMT == kFmtBF16) {
 #pragma unroll
       for (int s = 0; s < NSTEP; ++s) {
         const bf16x8 av = *reinterpret_cast<const bf16x8*>(arow + s * 16);
 #pragma unroll
         for (int g = 0; g < QG; ++g)
           acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, xr[g][s], acc[g], 0, 0, 0);
+      }
+    } else {
+      // Row scales ride in the MFMA (E8M0 per 32-element block): A's block is
+      // one train row, B's one query, so D is already |y|^2 - 2 x.y.
+      const int sa = ye_s[cur][c];
+#pragma unroll
+      for (int s = 0; s < NSTEP; ++s) {
+        const i32x4 lo = *reinterpret_cast<const i32x4*>(arow + s * 32);
+        const i32x4 hi = *reinterpret_cast<const i32x4*>(arow + s * 32 + 16);
+        const i32x8 av = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+#pragma unroll
+        for (int g = 0; g < QG; ++g)
+          acc[g] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, xr[g][s], acc[g], 0, 0, 0,
+                                                                   sa, 0, xsc[g]);
       }
     }
 
@@ -368,6 +416,14 @@ bool launch_mfma_search(const MfmaArgs& a, hipStream_t s) {
       case 32: launch_one<kFmtF32, 32, 2>(a, s); return true;
       case 64: launch_one<kFmtF32, 64, 2>(a, s); return true;
       case 128: launch_one<kFmtF32, 128, 2>(a, s); return true;
+      default: return false;
+    }
+  }
+  if (a.fmt == kFmtFP8) {
+    switch (a.dm) {
+      case 64: launch_one<kFmtFP8, 64, 2>(a, s); return true;
+      case 128: launch_one<kFmtFP8, 128, 2>(a, s); return true;
+      case 256: launch_one<kFmtFP8, 256, 2>(a, s); return true;
       default: return false;
     }
   }

@@ -96,7 +96,121 @@ __global__ __launch_bounds__(kThreads) void quantize_bf16_kernel(
   }
 }
 
+// OCP FP8 E4M3 (bias 7, no infinities, max normal 448), round to nearest
+// even, in software so the encoding does not depend on a conversion
+// instruction.  |v| <= 448 on entry.
+__device__ inline unsigned encode_e4m3(float v) {
+  const unsigned sign = v < 0.f ? 0x80u : 0u;
+  const float a = fabsf(v);
+  if (a < 0x1p-6f) {  // subnormal: m * 2^-9, m in 0..7 (8 rounds up to the min normal)
+    const unsigned m = (unsigned)rintf(a * 512.f);
+    return sign | m;  // m == 8 encodes exponent 1, mantissa 0: the min normal
+  }
+  int e;
+  const float fr = frexpf(a, &e);  // a = fr * 2^e, fr in [0.5, 1)
+  unsigned m = (unsigned)rintf((fr * 2.f - 1.f) * 8.f);
+  int ex = e - 1 + 7;
+  if (m == 8) {
+    m = 0;
+    ++ex;
+  }
+  if (ex > 15 || (ex == 15 && m > 6)) return sign | 0x7eu;  // saturate at 448
+  return sign | ((unsigned)ex << 3) | m;
+}
+
+__device__ inline float decode_e4m3(unsigned b) {
+  const unsigned ex = (b >> 3) & 15u, m = b & 7u;
+  const float a = ex == 0 ? (float)m * 0x1p-9f : ldexpf(1.f + (float)m * 0.125f, (int)ex - 7);
+  return (b & 0x80u) ? -a : a;
+}
+
+constexpr int kFp8MaxPerLane = 4;  // dm <= 256
+
+// One wave per row: centre, choose a power-of-two row scale 2^e with
+// |r - mu| / 2^e <= 448, encode, and keep |deq|^2 and the error bound.
+__global__ __launch_bounds__(kThreads) void quantize_fp8_kernel(
+    const float* __restrict__ src, int64_t n, int d, int dm, const float* __restrict__ mu,
+    unsigned char* __restrict__ dst, unsigned char* __restrict__ scale, float* __restrict__ norm,
+    float* __restrict__ err, DeviceScalars* sc, int track_max) {
+  __shared__ float wn[kThreads / 64];
+  __shared__ float we[kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float best_n = 0.f, best_e = 0.f;
+  for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
+    float cen[kFp8MaxPerLane];
+    float amax = 0.f;
+#pragma unroll
+    for (int i = 0; i < kFp8MaxPerLane; ++i) {
+      const int f = lane + 64 * i;
+      cen[i] = (f < d) ? src[r * d + f] - mu[f] : 0.f;
+      amax = fmaxf(amax, fabsf(cen[i]));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+    int e = 0;
+    if (amax > 0.f) {
+      int E;
+      const float fr = frexpf(amax / 448.f, &E);  // amax/448 = fr * 2^E
+      e = fr > 0.5f ? E : E - 1;                   // smallest e with amax/448 <= 2^e
+      e = e < -120 ? -120 : (e > 120 ? 120 : e);
+    }
+    float ns = 0.f;
+    double es = 0.0;
+#pragma unroll
+    for (int i = 0; i < kFp8MaxPerLane; ++i) {
+      const int f = lane + 64 * i;
+      if (f >= dm) break;
+      const unsigned b = f < d ? encode_e4m3(ldexpf(cen[i], -e)) : 0u;
+      dst[r * dm + f] = (unsigned char)b;
+      const float q = ldexpf(decode_e4m3(b), e);
+      ns = fmaf(q, q, ns);
+      if (f < d) {
+        const double ed = ((double)src[r * d + f] - (double)mu[f]) - (double)q;
+        es = fma(ed, ed, es);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ns += __shfl_xor(ns, o, 64);
+      es += __shfl_xor(es, o, 64);
+    }
+    const float eb = __uint_as_float(__float_as_uint((float)sqrt(es)) + 1u);
+    if (lane == 0) {
+      norm[r] = ns;
+      scale[r] = (unsigned char)(e + 127);  // E8M0
+      if (err) err[r] = eb;
+    }
+    best_n = fmaxf(best_n, ns);
+    best_e = fmaxf(best_e, eb);
+  }
+  if (!track_max) return;
+  if (lane == 0) {
+    wn[w] = best_n;
+    we[w] = best_e;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float mn = wn[0], me = we[0];
+    for (int i = 1; i < kThreads / 64; ++i) {
+      mn = fmaxf(mn, wn[i]);
+      me = fmaxf(me, we[i]);
+    }
+    atomicMax(&sc->max_ynorm_key, ordered_key((double)mn));
+    atomicMax(&sc->max_yerr_key, ordered_key((double)me));
+  }
+}
+
 }  // namespace
+
+void launch_quantize_fp8(const float* src, int64_t n, int d, int dm, const float* mu, void* dst,
+                         unsigned char* scale, float* norm, float* err, DeviceScalars* sc,
+                         bool track_max, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = ceil_div(n, 4) < 16384 ? ceil_div(n, 4) : 16384;
+  hipLaunchKernelGGL(quantize_fp8_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, src, n, d,
+                     dm, mu, reinterpret_cast<unsigned char*>(dst), scale, norm, err, sc,
+                     track_max ? 1 : 0);
+}
 
 void launch_column_mean(const float* Y, int64_t n_sample, int d, float* mu_sum, float* mu,
                         hipStream_t s) {

@@ -281,3 +281,41 @@ def test_bf16_gaussian_mixture_k32(require_gpu):
     # returned distances are exact FP64 recomputations of the returned rows
     got = oracle._direct_sqdist(X, Y, idx)
     np.testing.assert_allclose(dist, got, rtol=1e-6)
+
+
+@pytest.mark.parametrize("d", [64, 100, 256])
+def test_fp8_rerank_exact_when_certified(require_gpu, d):
+    """FP8 E4M3 candidates (power-of-two row scales) + exact re-rank."""
+    rs = np.random.RandomState(200 + d)
+    Y = rs.rand(30000, d).astype(np.float32)
+    X = rs.rand(300, d).astype(np.float32)
+    k = 10
+    (dist, idx), _, st = pa.parallel_knn(Y, X, k, return_distance=True, squared=True,
+                                         precision="fp8", return_stats=True)
+    assert st["kernel_path"] == 3
+    ref = oracle.exact_knn_f64(X, Y, k, return_boundary=True)
+    rec = oracle.recall_at_k(idx, ref[1])
+    _say(f"    fp8 d={d}: recall@{k}={rec:.5f} uncertified={st['n_uncertified']}")
+    assert rec >= 0.99
+    got = oracle._direct_sqdist(X, Y, idx)
+    np.testing.assert_allclose(dist, got, rtol=1e-6)
+    if st["n_uncertified"] == 0:
+        assert_knn_parity(X, Y, dist, idx, k, EPS_F32)
+
+
+def test_fp8_mixture_with_duplicates_k100(require_gpu):
+    """Mixture of 1,000 Gaussians (sigma 0.05) in d=256 with 1% duplicated
+    train rows, k=100, k'=400."""
+    rs = np.random.RandomState(11)
+    centres = rs.uniform(-1, 1, (1000, 256))
+    Y = _gaussian_mixture(rs, 40000, 256, centres, 0.05)
+    dup = rs.choice(40000, 400, replace=False)
+    Y[dup] = Y[rs.randint(0, 40000, 400)]
+    X = _gaussian_mixture(rs, 200, 256, centres, 0.05)
+    (dist, idx), _, st = pa.parallel_knn(Y, X, 100, return_distance=True, squared=True,
+                                         precision="fp8", oversample=400, return_stats=True)
+    ref = oracle.exact_knn_f64(X, Y, 100, return_boundary=True)
+    rec = oracle.recall_at_k(idx, ref[1])
+    _say(f"    fp8 mixture: recall@100={rec:.5f} uncertified={st['n_uncertified']}")
+    assert rec >= 0.99
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
