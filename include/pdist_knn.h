@@ -86,6 +86,40 @@ int knn_argkmin_f64(const double* X_train, int64_t n_train, const double* X_test
                     double* dist_out, int64_t* idx_out, void* workspace,
                     size_t workspace_bytes, void* stream, knn_stats* stats);
 
+/*
+ * Gradients of the k-NN distances returned by knn_argkmin_* with respect to
+ * the train and query rows.  No counterpart in the reference (its search is
+ * not differentiable); used by the torch.autograd.Function of
+ * pdist_aggregation_amd.autograd.
+ *
+ * indices: the forward's (n_test x k) result; grad_dist: dL/d(dist), same
+ * shape; flags: KNN_FLAG_SQUARED if the forward returned squared distances.
+ * grad_X_train (n_train x dim) and/or grad_X_test (n_test x dim) may be NULL.
+ * grad_X_train is overwritten (zeroed, then accumulated with atomics: the
+ * summation order, hence the last bits, is not deterministic).
+ */
+int knn_argkmin_backward_f32(const float* X_train, int64_t n_train, const float* X_test,
+                             int64_t n_test, int64_t dim, int64_t k, const int64_t* indices,
+                             const float* grad_dist, int flags, float* grad_X_train,
+                             float* grad_X_test, void* stream);
+
+int knn_argkmin_backward_f64(const double* X_train, int64_t n_train, const double* X_test,
+                             int64_t n_test, int64_t dim, int64_t k, const int64_t* indices,
+                             const double* grad_dist, int flags, double* grad_X_train,
+                             double* grad_X_test, void* stream);
+
+/*
+ * Dense squared-distance matrix out[i][j] = sum_f (X_test[i][f] -
+ * X_train[j][f])^2, (n_test x n_train) row-major, by direct differences.
+ * Replaces the SYCL backend's distance_squared_matrix / pw_d2m
+ * (sycl_ext/_pdist_aggregation_sycl.pyx:73-99,
+ * sycl_ext/parallel_knn_helper.hpp:26-55).
+ */
+int knn_sqdist_matrix_f32(const float* X_train, int64_t n_train, const float* X_test,
+                          int64_t n_test, int64_t dim, float* out, void* stream);
+int knn_sqdist_matrix_f64(const double* X_train, int64_t n_train, const double* X_test,
+                          int64_t n_test, int64_t dim, double* out, void* stream);
+
 /* dtype_bytes: 4 (f32) or 8 (f64).  Returns 0 for an invalid request. */
 size_t knn_argkmin_workspace_size(int dtype_bytes, int64_t n_train, int64_t n_test, int64_t dim,
                                   int64_t k, int precision, int64_t oversample);

@@ -28,6 +28,10 @@ KERNEL_PATH = {0: "generic-simt", 1: "mfma-fp32", 2: "mfma-bf16", 3: "mfma-fp8"}
 EXPORTED = (
     "knn_argkmin_f32",
     "knn_argkmin_f64",
+    "knn_argkmin_backward_f32",
+    "knn_argkmin_backward_f64",
+    "knn_sqdist_matrix_f32",
+    "knn_sqdist_matrix_f64",
     "knn_argkmin_workspace_size",
     "knn_last_error",
     "knn_abi_version",
@@ -81,6 +85,16 @@ def lib():
             fn = getattr(h, name)
             fn.restype = ctypes.c_int
             fn.argtypes = _argkmin_argtypes()
+        for name in ("knn_argkmin_backward_f32", "knn_argkmin_backward_f64"):
+            fn = getattr(h, name)
+            fn.restype = ctypes.c_int
+            v, i64 = ctypes.c_void_p, ctypes.c_int64
+            fn.argtypes = [v, i64, v, i64, i64, i64, v, v, ctypes.c_int, v, v, v]
+        for name in ("knn_sqdist_matrix_f32", "knn_sqdist_matrix_f64"):
+            fn = getattr(h, name)
+            fn.restype = ctypes.c_int
+            v, i64 = ctypes.c_void_p, ctypes.c_int64
+            fn.argtypes = [v, i64, v, i64, i64, v, v]
         h.knn_argkmin_workspace_size.restype = ctypes.c_size_t
         h.knn_argkmin_workspace_size.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
                                                  ctypes.c_int64, ctypes.c_int64, ctypes.c_int,

@@ -417,9 +417,80 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   return KNN_OK;
 }
 
+template <class T>
+int run_backward(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test,
+                 int64_t dim, int64_t k, const int64_t* indices, const T* grad_dist, int flags,
+                 T* grad_X_train, T* grad_X_test, void* stream_ptr) {
+  g_last_error.clear();
+  if (n_train <= 0 || n_test < 0 || dim <= 0 || k <= 0)
+    return fail(KNN_EINVAL, "n_train, dim and k must be positive and n_test non-negative");
+  if (dim > INT32_MAX || k > INT32_MAX) return fail(KNN_EUNSUPPORTED, "dim or k too large");
+  if (!X_train || (n_test > 0 && (!X_test || !indices || !grad_dist)))
+    return fail(KNN_EINVAL, "null array pointer");
+  if (!grad_X_train && !grad_X_test) return fail(KNN_EINVAL, "no gradient requested");
+  knn::BackwardArgs<T> a;
+  a.X = X_test;
+  a.Y = X_train;
+  a.n_q = n_test;
+  a.n_t = n_train;
+  a.d = (int)dim;
+  a.k = (int)k;
+  a.idx = indices;
+  a.grad = grad_dist;
+  a.squared = (flags & KNN_FLAG_SQUARED) ? 1 : 0;
+  a.grad_X = grad_X_test;
+  a.grad_Y = grad_X_train;
+  knn::launch_backward<T>(a, reinterpret_cast<hipStream_t>(stream_ptr));
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(KNN_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return KNN_OK;
+}
+
+template <class T>
+int run_sqdist(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int64_t dim,
+               T* out, void* stream_ptr) {
+  g_last_error.clear();
+  if (n_train < 0 || n_test < 0 || dim <= 0)
+    return fail(KNN_EINVAL, "dim must be positive and the row counts non-negative");
+  if (dim > INT32_MAX) return fail(KNN_EUNSUPPORTED, "dim too large");
+  if (n_train > 0 && n_test > 0 && (!X_train || !X_test || !out))
+    return fail(KNN_EINVAL, "null array pointer");
+  knn::launch_sqdist<T>(X_test, n_test, X_train, n_train, (int)dim, out,
+                        reinterpret_cast<hipStream_t>(stream_ptr));
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(KNN_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return KNN_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int knn_sqdist_matrix_f32(const float* X_train, int64_t n_train, const float* X_test,
+                          int64_t n_test, int64_t dim, float* out, void* stream) {
+  return run_sqdist<float>(X_train, n_train, X_test, n_test, dim, out, stream);
+}
+
+int knn_sqdist_matrix_f64(const double* X_train, int64_t n_train, const double* X_test,
+                          int64_t n_test, int64_t dim, double* out, void* stream) {
+  return run_sqdist<double>(X_train, n_train, X_test, n_test, dim, out, stream);
+}
+
+int knn_argkmin_backward_f32(const float* X_train, int64_t n_train, const float* X_test,
+                             int64_t n_test, int64_t dim, int64_t k, const int64_t* indices,
+                             const float* grad_dist, int flags, float* grad_X_train,
+                             float* grad_X_test, void* stream) {
+  return run_backward<float>(X_train, n_train, X_test, n_test, dim, k, indices, grad_dist, flags,
+                             grad_X_train, grad_X_test, stream);
+}
+
+int knn_argkmin_backward_f64(const double* X_train, int64_t n_train, const double* X_test,
+                             int64_t n_test, int64_t dim, int64_t k, const int64_t* indices,
+                             const double* grad_dist, int flags, double* grad_X_train,
+                             double* grad_X_test, void* stream) {
+  return run_backward<double>(X_train, n_train, X_test, n_test, dim, k, indices, grad_dist,
+                              flags, grad_X_train, grad_X_test, stream);
+}
 
 int knn_argkmin_f32(const float* X_train, int64_t n_train, const float* X_test, int64_t n_test,
                     int64_t dim, int64_t k, int64_t chunk_size, int strategy, int precision,

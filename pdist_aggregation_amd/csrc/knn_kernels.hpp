@@ -135,6 +135,28 @@ struct MergeArgs {
 template <class T>
 void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s);
 
+// Backward of the k-NN distances (knn_backward.hip).
+template <class T>
+struct BackwardArgs {
+  const T* X;  // queries (n_q x d)
+  const T* Y;  // train rows (n_t x d)
+  int64_t n_q, n_t;
+  int d, k;
+  const int64_t* idx;  // forward indices (n_q x k); entries outside [0, n_t) are skipped
+  const T* grad;       // dL/d(dist) (n_q x k)
+  int squared;         // distances were squared (else Euclidean)
+  T* grad_X;           // n_q x d, or nullptr
+  T* grad_Y;           // n_t x d, or nullptr (zeroed, then scattered with atomics)
+};
+
+template <class T>
+void launch_backward(const BackwardArgs<T>& a, hipStream_t s);
+
+// Dense direct-difference squared distances, out is n_q x n_t (knn_dense.hip).
+template <class T>
+void launch_sqdist(const T* X, int64_t n_q, const T* Y, int64_t n_t, int d, T* out,
+                   hipStream_t s);
+
 size_t merge_lds_bytes(int n_splits, int kc, size_t elem);
 
 }  // namespace knn

@@ -164,3 +164,34 @@ def parallel_knn(X_train, X_test, k, chunk_size=CHUNK_SIZE, strategy="auto",
         st["strategy_used"] = _capi.STRATEGY_NAME.get(st["strategy_used"], "?")
         return out + (st,)
     return out
+
+
+def sqdist_matrix(X_train, X_test, stream=None):
+    """Dense (n_test x n_train) squared distances by direct differences.
+
+    The SYCL backend's ``distance_squared_matrix``
+    (sycl_ext/_pdist_aggregation_sycl.pyx:73-99) on the MI355X.  numpy in,
+    numpy out; GPU tensors in, GPU tensor out.
+    """
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError("pdist_aggregation_amd needs a visible AMD GPU (no CPU fallback)")
+    lib = _capi.lib()
+    device = X_train.device if isinstance(X_train, torch.Tensor) and X_train.is_cuda \
+        else torch.device("cuda", torch.cuda.current_device())
+    Ytr, _ = _as_device(X_train, "X_train", torch, device)
+    Xte, _ = _as_device(X_test, "X_test", torch, device)
+    if Ytr.dtype != Xte.dtype or Ytr.dtype not in (torch.float32, torch.float64):
+        raise TypeError("X_train and X_test must both be float32 or both float64")
+    if Ytr.shape[1] != Xte.shape[1]:
+        raise ValueError("Dimensionality of test and train datasets must be the same")
+    out = torch.empty((Xte.shape[0], Ytr.shape[0]), dtype=Ytr.dtype, device=device)
+    s = stream if stream is not None else torch.cuda.current_stream(device)
+    fn = lib.knn_sqdist_matrix_f64 if Ytr.dtype == torch.float64 else lib.knn_sqdist_matrix_f32
+    with torch.cuda.device(device):
+        code = fn(Ytr.data_ptr(), Ytr.shape[0], Xte.data_ptr(), Xte.shape[0], Ytr.shape[1],
+                  out.data_ptr(), ctypes.c_void_p(s.cuda_stream))
+    _capi.check(code)
+    if not (isinstance(X_test, torch.Tensor) and X_test.is_cuda):
+        return out.cpu().numpy()
+    return out

@@ -11,6 +11,7 @@ here.
 from __future__ import annotations
 
 from .knn import parallel_knn as _parallel_knn
+from .knn import sqdist_matrix as _sqdist_matrix
 
 
 def parallel_knn(X_train, X_test, k, queue=None, strategy="auto", return_distance=False):
@@ -23,6 +24,13 @@ def parallel_knn(X_train, X_test, k, queue=None, strategy="auto", return_distanc
     (dist, idx), _ = _parallel_knn(X_train, X_test, k, strategy="auto", return_distance=True,
                                    squared=True, stream=queue)
     return (idx, dist) if return_distance else idx
+
+
+def distance_squared_matrix(X_train, X_test, queue=None):
+    """(n_test x n_train) squared distances (:73-99), direct differences."""
+    if X_train.shape[1] != X_test.shape[1]:
+        raise ValueError("Dimensionality of test and train datasets must be the same")
+    return _sqdist_matrix(X_train, X_test, stream=queue)
 
 
 class NearestNeighbors:

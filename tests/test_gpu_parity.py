@@ -319,3 +319,20 @@ def test_fp8_mixture_with_duplicates_k100(require_gpu):
     _say(f"    fp8 mixture: recall@100={rec:.5f} uncertified={st['n_uncertified']}")
     assert rec >= 0.99
     assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [1, 7, 50, 129])
+def test_distance_squared_matrix(require_gpu, dtype, d):
+    """Dense squared distances (the SYCL backend's distance_squared_matrix),
+    direct differences, against numpy in FP64 (sycl_ext/check_distances.py
+    compares the same matrix with numpy)."""
+    from pdist_aggregation_amd import sycl_compat
+
+    rs = np.random.RandomState(d)
+    Y = rs.rand(517, d).astype(dtype)
+    X = rs.rand(131, d).astype(dtype)
+    D = sycl_compat.distance_squared_matrix(Y, X)
+    assert D.shape == (131, 517) and D.dtype == dtype
+    ref = ((X.astype(np.float64)[:, None, :] - Y.astype(np.float64)[None]) ** 2).sum(-1)
+    np.testing.assert_allclose(D, ref, rtol=1e-5 if dtype == np.float32 else 1e-13, atol=1e-12)
