@@ -134,6 +134,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -157,6 +160,14 @@ def main():
     Xd = torch.from_numpy(X).to(dev)
     flops = 2.0 * args.nq * args.nt * args.dim
     print(f"[bench] inputs ready: X {X.shape} Y {Y.shape}", file=sys.stderr, flush=True)
+    if args.copy_mb > 0:
+        src = torch.empty(args.copy_mb << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print(f"[bench] copied {args.copy_mb} MiB device to device", file=sys.stderr, flush=True)
 
     # FP64 exact answer of the same inputs (the library's float64 path, itself
     # checked against the CPU oracle by the test suite) for recall@k.

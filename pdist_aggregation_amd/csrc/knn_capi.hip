@@ -75,7 +75,8 @@ struct Plan {
          off_xerr = 0, off_xscale = 0, off_yscale = 0, total = 0;
 };
 
-constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3;
+constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
+              kPathLowd = 4;
 
 // Split count for a grid of n_qblocks query blocks: enough workgroups to keep
 // every CU busy several times over, each split at least min_steps row steps,
@@ -139,6 +140,8 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
              mfma_cap_ok) {
     p.path = kPathMfmaF32;
     p.dm = std::max(32, knn::next_pow2((int)dim));
+  } else if (dim < 16 && mfma_cap_ok) {
+    p.path = kPathLowd;
   }
 
   int bq, step;
@@ -146,6 +149,10 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     bq = knn::kGenQ;
     step = knn::kGenY;
     p.cap = knn::next_pow2((int)kc + knn::kGenY);
+  } else if (p.path == kPathLowd) {
+    bq = knn::kLowdQueriesPerBlock;
+    step = knn::kLowdRowStep;
+    p.cap = mfma_cap;
   } else {
     bq = knn::kMfmaQueriesPerBlock;
     step = knn::kMfmaRowStep;
@@ -160,7 +167,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   // An evenly divisible split count keeps the XCD-aware block order.
   if (p.path != kPathGeneric && p.n_splits != splits) p.n_splits = splits;
 
-  const size_t celem = p.path == kPathGeneric ? elem : sizeof(float);
+  const size_t celem = (p.path == kPathGeneric || p.path == kPathLowd) ? elem : sizeof(float);
   const int64_t nq1 = std::max<int64_t>(n_test, 1);
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -276,7 +283,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.xerr = nullptr;
   ma.unit_roundoff = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
   ma.acc_terms = 2 * (int)dim + 4;
-  if (p.path == kPathGeneric) {
+  if (p.path == kPathGeneric || p.path == kPathLowd) {
     // 2. norms  3. search
     knn::launch_row_norms<T>(X_train, n_train, (int)dim, ynorm, sc, stream);
     knn::SearchArgs<T> sa;
@@ -289,6 +296,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     sa.kc = p.kc;
     sa.cap = p.cap;
     sa.n_splits = p.n_splits;
+    sa.n_qblocks = (int)p.n_qblocks;
     sa.rows_per_split = p.rows_per_split;
     sa.cand_d = reinterpret_cast<T*>(ws + p.off_cand_d);
     sa.cand_i = reinterpret_cast<int*>(ws + p.off_cand_i);
@@ -296,7 +304,10 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     sa.part_d = reinterpret_cast<T*>(ws + p.off_part_d);
     sa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
     timer.mark(1);
-    knn::launch_generic_search<T>(sa, stream);
+    if (p.path == kPathLowd)
+      knn::launch_lowd_search<T>(sa, stream);
+    else
+      knn::launch_generic_search<T>(sa, stream);
     timer.mark(2);
     ma.part_d = sa.part_d;
     ma.part_i = sa.part_i;

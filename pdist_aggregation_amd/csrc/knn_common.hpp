@@ -43,6 +43,17 @@ __device__ inline double pos_inf<double>() {
 }
 
 template <class T>
+__device__ inline T max_finite();
+template <>
+__device__ inline float max_finite<float>() {
+  return FLT_MAX;
+}
+template <>
+__device__ inline double max_finite<double>() {
+  return DBL_MAX;
+}
+
+template <class T>
 __device__ inline bool key_less(T da, int ia, T db, int ib) {
   return da < db || (da == db && ia < ib);
 }
@@ -74,6 +85,55 @@ __device__ void block_bitonic_sort(T* sd, int* si, int n) {
       __syncthreads();
     }
   }
+}
+
+// Orders LDS (and same-CU global) accesses between the lanes of one wave.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__host__ __device__ inline int next_pow2(int v);
+
+// Wave-cooperative sort of one query's candidate list (cd, ci: n entries in
+// global scratch) through wave-private LDS scratch (sd, si: >= next_pow2(n)
+// entries), keeping the best min(n, kc) entries in cd/ci.  Called by all 64
+// lanes of one wave with the same arguments; no workgroup barrier.
+template <class T>
+__device__ void wave_sort_cut(T* cd, int* ci, int n, int kc, T* sd, int* si, int lane) {
+  const int P = next_pow2(n);
+  for (int e = lane; e < P; e += 64) {
+    const bool ok = e < n;
+    sd[e] = ok ? cd[e] : pos_inf<T>();
+    si[e] = ok ? ci[e] : INT_MAX;
+  }
+  wave_sync();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = lane; i < (P >> 1); i += 64) {
+        const int lo = 2 * stride * (i / stride) + (i % stride);
+        const int hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const T dl = sd[lo], dh = sd[hi];
+        const int il = si[lo], ih = si[hi];
+        const bool swap = up ? key_less(dh, ih, dl, il) : key_less(dl, il, dh, ih);
+        if (swap) {
+          sd[lo] = dh;
+          sd[hi] = dl;
+          si[lo] = ih;
+          si[hi] = il;
+        }
+      }
+      wave_sync();
+    }
+  }
+  const int m = n < kc ? n : kc;
+  for (int e = lane; e < m; e += 64) {
+    cd[e] = sd[e];
+    ci[e] = si[e];
+  }
+  wave_sync();
 }
 
 __host__ __device__ inline int next_pow2(int v) {

@@ -38,6 +38,7 @@ struct SearchArgs {
   int kc;                  // candidates kept per query (k')
   int cap;                 // per-query capacity of the candidate scratch (power of two)
   int n_splits;
+  int n_qblocks;           // query blocks (low-d kernel grid)
   int64_t rows_per_split;  // multiple of the kernel's row step
   T* cand_d;               // [blocks][queries per block][cap]
   int* cand_i;
@@ -106,6 +107,15 @@ void launch_row_norms(const T* Y, int64_t n, int d, T* out, DeviceScalars* sc, h
 
 template <class T>
 void launch_generic_search(const SearchArgs<T>& a, hipStream_t s);
+
+// Low-dimensional (d < 16) VALU kernel (knn_lowd.hip): 256 queries per
+// workgroup, 32 rows per step.  Returns false for d >= 16.
+constexpr int kLowdQueriesPerBlock = 256;
+constexpr int kLowdRowStep = 32;
+template <class T>
+using LowdArgs = SearchArgs<T>;
+template <class T>
+bool launch_lowd_search(const LowdArgs<T>& a, hipStream_t s);
 
 // Merge the per-split candidate lists, re-rank them with exact FP64
 // direct-difference distances from the original rows, keep k.

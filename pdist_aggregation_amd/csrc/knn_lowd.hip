@@ -1,0 +1,183 @@
+// Fused distance + candidate kernel for low-dimensional points (d < 16),
+// float or double, VALU FMA.
+//
+// At d = 3 a GEMM formulation wastes the matrix cores (K would be padded
+// 3 -> 32) and the generic kernel's 16-feature LDS slices waste 13/16 of its
+// FMAs.  Here every lane owns ONE query whose -2 x sits in registers; the
+// workgroup stages 32 train rows per step in LDS and every lane walks all 32
+// rows (broadcast LDS reads), so a candidate costs d FMAs plus its share of
+// a min-tree.  Selection is the same contract as the MFMA kernel: append
+// v = |y|^2 - 2 x.y <= tau to the query's list in global scratch, compact
+// with a wave-local sort, share tau across splits with atomicMin.
+#include "knn_common.hpp"
+#include "knn_kernels.hpp"
+
+namespace knn {
+
+namespace {
+
+constexpr int kRows = 32;  // train rows per step
+
+template <class T, int DP>
+__global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
+  __shared__ T ys[2][kRows][DP];
+  __shared__ T yn_s[2][kRows];
+  __shared__ int cnt_s[kThreads];
+  extern __shared__ __align__(16) unsigned char dyn_smem[];  // 4 waves x cap x (T + int)
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int qb, split;
+  {
+    const int b = blockIdx.x;
+    if ((a.n_splits & 7) == 0) {
+      const int xcd = b & 7, t = b >> 3;
+      split = xcd + 8 * (t / a.n_qblocks);
+      qb = t % a.n_qblocks;
+    } else {
+      split = b / a.n_qblocks;
+      qb = b % a.n_qblocks;
+    }
+  }
+  const int64_t q0 = (int64_t)qb * kThreads;
+  const int64_t q = q0 + tid;
+  const bool qok = q < a.n_q;
+  const int64_t ybeg = (int64_t)split * a.rows_per_split;
+  const int64_t yend = ybeg + a.rows_per_split < a.n_t ? ybeg + a.rows_per_split : a.n_t;
+  const size_t blk = (size_t)split * a.n_qblocks + qb;
+  T* cand_d = a.cand_d + blk * kThreads * a.cap;
+  int* cand_i = a.cand_i + blk * kThreads * a.cap;
+  T* sd = reinterpret_cast<T*>(dyn_smem + (size_t)w * a.cap * (sizeof(T) + sizeof(int)));
+  int* si = reinterpret_cast<int*>(sd + a.cap);
+  T* my_d = cand_d + (size_t)tid * a.cap;
+  int* my_i = cand_i + (size_t)tid * a.cap;
+
+  T xm2[DP];
+#pragma unroll
+  for (int f = 0; f < DP; ++f) xm2[f] = (qok && f < a.d) ? T(-2) * a.X[q * a.d + f] : T(0);
+  T tau = qok ? max_finite<T>() : -pos_inf<T>();  // +inf marks rows past the end
+  cnt_s[tid] = 0;
+
+  // tile loader: rows x DP elements (zero padded) + norms (+inf past the end)
+  T pre[(kRows * DP + kThreads - 1) / kThreads];
+  T pre_n = T(0);
+  auto load_tile = [&](int64_t y0) {
+#pragma unroll
+    for (int i = 0; i < (kRows * DP + kThreads - 1) / kThreads; ++i) {
+      const int e = tid + i * kThreads;
+      const int r = e / DP, f = e % DP;
+      const int64_t y = y0 + r;
+      pre[i] = (e < kRows * DP && y < yend && f < a.d) ? a.Y[y * a.d + f] : T(0);
+    }
+    if (tid < kRows) pre_n = (y0 + tid < yend) ? a.ynorm[y0 + tid] : pos_inf<T>();
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < (kRows * DP + kThreads - 1) / kThreads; ++i) {
+      const int e = tid + i * kThreads;
+      if (e < kRows * DP) ys[buf][e / DP][e % DP] = pre[i];
+    }
+    if (tid < kRows) yn_s[buf][tid] = pre_n;
+  };
+  load_tile(ybeg);
+  store_tile(0);
+  __syncthreads();
+
+  int cur = 0, step = 0;
+  for (int64_t y0 = ybeg; y0 < yend; y0 += kRows, ++step) {
+    const bool more = y0 + kRows < yend;
+    if (more) load_tile(y0 + kRows);
+    T v[kRows];
+    T m = pos_inf<T>();
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      T acc = yn_s[cur][r];
+#pragma unroll
+      for (int f = 0; f < DP; ++f) acc = fma(xm2[f], ys[cur][r][f], acc);
+      v[r] = acc;
+      m = m < acc ? m : acc;
+    }
+    bool overflow = false;
+    if (__any(m <= tau)) {
+      if (m <= tau) {
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+          if (v[r] <= tau) {
+            const int pos = cnt_s[tid]++;  // this lane owns the query: no atomic
+            my_d[pos] = v[r];
+            my_i[pos] = (int)(y0 + r);
+          }
+        }
+        overflow = cnt_s[tid] > a.cap - kRows;
+      }
+    }
+    if (__any(overflow)) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      wave_sync();
+      for (int qq = 0; qq < 64; ++qq) {
+        const int ql = w * 64 + qq;
+        const int n = cnt_s[ql];
+        if (n <= a.cap - kRows) continue;
+        T* cd = cand_d + (size_t)ql * a.cap;
+        int* ci = cand_i + (size_t)ql * a.cap;
+        wave_sort_cut(cd, ci, n, a.kc, sd, si, lane);
+        if (lane == 0) {
+          cnt_s[ql] = n < a.kc ? n : a.kc;
+          if (n >= a.kc) atomicMin(&a.tau[q0 + ql], ordered_key((double)sd[a.kc - 1]));
+        }
+        if (lane == qq && n >= a.kc) tau = tau < sd[a.kc - 1] ? tau : sd[a.kc - 1];
+        wave_sync();
+      }
+    }
+    if ((step & 15) == 15 && qok) {
+      const T g = (T)from_ordered_key(__atomic_load_n(&a.tau[q], __ATOMIC_RELAXED));
+      tau = g < tau ? g : tau;
+    }
+    if (more) store_tile(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  wave_sync();
+  for (int qq = 0; qq < 64; ++qq) {
+    const int ql = w * 64 + qq;
+    const int n = cnt_s[ql];
+    if (n <= 0) continue;
+    wave_sort_cut(cand_d + (size_t)ql * a.cap, cand_i + (size_t)ql * a.cap, n, a.kc, sd, si, lane);
+    if (lane == 0) cnt_s[ql] = n < a.kc ? n : a.kc;
+    wave_sync();
+  }
+  for (int qq = 0; qq < 64; ++qq) {
+    const int ql = w * 64 + qq;
+    if (q0 + ql >= a.n_q) break;
+    const int n = cnt_s[ql];
+    for (int j = lane; j < a.kc; j += 64) {
+      const size_t o = ((size_t)(q0 + ql) * a.n_splits + split) * a.kc + j;
+      a.part_d[o] = j < n ? cand_d[(size_t)ql * a.cap + j] : pos_inf<T>();
+      a.part_i[o] = j < n ? cand_i[(size_t)ql * a.cap + j] : -1;
+    }
+  }
+}
+
+template <class T, int DP>
+void launch_dp(const LowdArgs<T>& a, hipStream_t s) {
+  const size_t lds = (size_t)4 * a.cap * (sizeof(T) + sizeof(int));
+  hipLaunchKernelGGL((lowd_search_kernel<T, DP>), dim3((unsigned)(a.n_qblocks * a.n_splits)),
+                     dim3(kThreads), lds, s, a);
+}
+
+}  // namespace
+
+template <class T>
+bool launch_lowd_search(const LowdArgs<T>& a, hipStream_t s) {
+  if (a.d <= 4) launch_dp<T, 4>(a, s);
+  else if (a.d <= 8) launch_dp<T, 8>(a, s);
+  else if (a.d <= 16) launch_dp<T, 16>(a, s);
+  else return false;
+  return true;
+}
+
+template bool launch_lowd_search<float>(const LowdArgs<float>&, hipStream_t);
+template bool launch_lowd_search<double>(const LowdArgs<double>&, hipStream_t);
+
+}  // namespace knn

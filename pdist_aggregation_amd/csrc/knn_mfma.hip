@@ -43,12 +43,6 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBY = 32;  // train rows per step (one MFMA row block)
 
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 template <int FMT>
 struct Fmt;
 template <>
@@ -75,43 +69,6 @@ struct Fmt<kFmtFP8> {
 template <int FMT, int DM>
 constexpr int row_pitch() {
   return DM * Fmt<FMT>::esize + 16;
-}
-
-// Wave-cooperative sort + cut of one query's list (wave-local LDS scratch).
-__device__ void wave_sort_cut(float* cd, int* ci, int n, int kc, float* sd, int* si,
-                              int lane) {
-  const int P = next_pow2(n);
-  for (int e = lane; e < P; e += 64) {
-    const bool ok = e < n;
-    sd[e] = ok ? cd[e] : __builtin_huge_valf();
-    si[e] = ok ? ci[e] : INT_MAX;
-  }
-  wave_sync();
-  for (int size = 2; size <= P; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = lane; i < (P >> 1); i += 64) {
-        const int lo = 2 * stride * (i / stride) + (i % stride);
-        const int hi = lo + stride;
-        const bool up = (lo & size) == 0;
-        const float dl = sd[lo], dh = sd[hi];
-        const int il = si[lo], ih = si[hi];
-        const bool swap = up ? key_less(dh, ih, dl, il) : key_less(dl, il, dh, ih);
-        if (swap) {
-          sd[lo] = dh;
-          sd[hi] = dl;
-          si[lo] = ih;
-          si[hi] = il;
-        }
-      }
-      wave_sync();
-    }
-  }
-  const int m = n < kc ? n : kc;
-  for (int e = lane; e < m; e += 64) {
-    cd[e] = sd[e];
-    ci[e] = si[e];
-  }
-  wave_sync();
 }
 
 template <int FMT, int DM, int QG>
