@@ -10,6 +10,7 @@
 // (src/_pdist_aggregation.pyx:236-448) or as a queue of SYCL kernels per
 // chunk pair (sycl_ext/parallel_knn.hpp:337-534).
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -91,6 +92,38 @@ int choose_splits(int64_t n_qblocks, int64_t n_train, int row_step, int kc, int6
   return (int)std::max<int64_t>(splits, 1);
 }
 
+// Compute units of the current device (256 on an MI355X; also the fallback
+// when no device is visible, so plans made without a GPU match one).
+int device_cus() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+    return n;
+  return 256;
+}
+
+// Split count for the one-wave-of-workgroups kernels (MFMA, low-d): run time
+// is proportional to rounds(s) / s with rounds(s) = ceil(n_qblocks * s /
+// slots); take the smallest s within 4% of the best, since every split adds
+// its own warm-up and k' candidates to merge.  PDIST_KNN_SPLITS overrides.
+int choose_splits_slots(int64_t n_qblocks, int64_t n_train, int row_step, int kc,
+                        int64_t slots) {
+  if (const char* env = std::getenv("PDIST_KNN_SPLITS")) {
+    const int v = std::atoi(env);
+    if (v > 0) return v;
+  }
+  int64_t smax = std::min<int64_t>(n_train / (8 * row_step), knn::kMaxMerge / kc);
+  smax = std::max<int64_t>(1, std::min<int64_t>(smax, 512));
+  auto cost = [&](int64_t s) {
+    return (double)knn::ceil_div(n_qblocks * s, std::max<int64_t>(slots, 1)) / (double)s;
+  };
+  double best = cost(1);
+  for (int64_t s = 2; s <= smax; ++s) best = std::min(best, cost(s));
+  for (int64_t s = 1; s <= smax; ++s)
+    if (cost(s) <= 1.04 * best) return (int)s;
+  return 1;
+}
+
 int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim, int64_t k,
               int precision, int64_t oversample, bool force_generic = false) {
   if (n_train <= 0 || n_test < 0 || dim <= 0)
@@ -127,20 +160,21 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   p.elem = elem;
   p.kc = (int)kc;
 
-  const int mfma_cap = knn::next_pow2((int)kc + knn::kMfmaRowStep);
+  const int mfma_cap = knn::next_pow2((int)kc + knn::kMfmaCapSlack);
   const bool mfma_cap_ok = mfma_cap <= knn::kMfmaMaxCap;
+  const int lowd_cap = knn::next_pow2((int)kc + knn::kLowdRowStep);
   p.path = kPathGeneric;
   if (precision != KNN_PRECISION_EXACT) {
     if (dim > 256 || !mfma_cap_ok)
       return fail(KNN_EUNSUPPORTED, "bf16/fp8 search supports dim <= 256 and k' <= " +
-                                        std::to_string(knn::kMfmaMaxCap - knn::kMfmaRowStep));
+                                        std::to_string(knn::kMfmaMaxCap - knn::kMfmaCapSlack));
     p.path = precision == KNN_PRECISION_BF16 ? kPathMfmaBF16 : kPathMfmaFP8;
     p.dm = std::max(p.path == kPathMfmaBF16 ? 32 : 64, knn::next_pow2((int)dim));
   } else if (!force_generic && elem == 4 && dim % 4 == 0 && dim >= 16 && dim <= 128 &&
              mfma_cap_ok) {
     p.path = kPathMfmaF32;
     p.dm = std::max(32, knn::next_pow2((int)dim));
-  } else if (dim < 16 && mfma_cap_ok) {
+  } else if (dim < 16 && lowd_cap <= knn::kMfmaMaxCap) {
     p.path = kPathLowd;
   }
 
@@ -152,20 +186,34 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   } else if (p.path == kPathLowd) {
     bq = knn::kLowdQueriesPerBlock;
     step = knn::kLowdRowStep;
-    p.cap = mfma_cap;
+    p.cap = lowd_cap;
   } else {
-    bq = knn::kMfmaQueriesPerBlock;
+    const int fmt = p.path == kPathMfmaF32    ? knn::kFmtF32
+                    : p.path == kPathMfmaBF16 ? knn::kFmtBF16
+                                              : knn::kFmtFP8;
+    bq = knn::mfma_block_queries(fmt, p.dm);
     step = knn::kMfmaRowStep;
     p.cap = mfma_cap;
   }
   p.n_qblocks = knn::ceil_div(std::max<int64_t>(n_test, 1), bq);
-  const int splits = choose_splits(p.n_qblocks, n_train, step, p.kc,
-                                   p.path == kPathGeneric ? 2048 : 1024,
-                                   p.path != kPathGeneric);
+  int splits;
+  if (p.path == kPathGeneric) {
+    splits = choose_splits(p.n_qblocks, n_train, step, p.kc, 2048, false);
+  } else {
+    int per_cu = 2;
+    if (p.path == kPathLowd)
+      per_cu = elem == 4 ? knn::lowd_blocks_per_cu<float>((int)dim, p.cap)
+                         : knn::lowd_blocks_per_cu<double>((int)dim, p.cap);
+    else
+      per_cu = knn::mfma_blocks_per_cu(p.path == kPathMfmaF32    ? knn::kFmtF32
+                                       : p.path == kPathMfmaBF16 ? knn::kFmtBF16
+                                                                 : knn::kFmtFP8,
+                                       p.dm, p.cap);
+    splits = choose_splits_slots(p.n_qblocks, n_train, step, p.kc,
+                                 (int64_t)device_cus() * per_cu);
+  }
   p.rows_per_split = knn::ceil_div(knn::ceil_div(n_train, splits), step) * step;
   p.n_splits = (int)knn::ceil_div(n_train, p.rows_per_split);
-  // An evenly divisible split count keeps the XCD-aware block order.
-  if (p.path != kPathGeneric && p.n_splits != splits) p.n_splits = splits;
 
   const size_t celem = (p.path == kPathGeneric || p.path == kPathLowd) ? elem : sizeof(float);
   const int64_t nq1 = std::max<int64_t>(n_test, 1);

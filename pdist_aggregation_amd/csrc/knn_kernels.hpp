@@ -53,9 +53,11 @@ struct SearchArgs {
 constexpr int kFmtF32 = 0;   // v_mfma_f32_32x32x2_f32 on the caller's rows
 constexpr int kFmtBF16 = 1;  // v_mfma_f32_32x32x16_bf16 on quantised rows
 constexpr int kFmtFP8 = 2;   // v_mfma_scale_f32_32x32x64_f8f6f4, E4M3 + row scales
-constexpr int kMfmaQueriesPerBlock = 256;  // 4 waves x 2 groups x 32 queries
 constexpr int kMfmaRowStep = 32;
 constexpr int kMfmaMaxCap = 512;  // per-query list capacity (wave sort in LDS)
+// A query's list is two halves of cap/2, one per owning lane; each half must
+// hold ceil(k'/2) survivors plus one step's 16 appends: cap >= k' + 34.
+constexpr int kMfmaCapSlack = 34;
 
 struct MfmaArgs {
   int fmt;
@@ -80,6 +82,10 @@ struct MfmaArgs {
 
 // Returns false when no instantiation matches (fmt, dm).
 bool launch_mfma_search(const MfmaArgs& a, hipStream_t s);
+// Queries per workgroup of the (fmt, dm) instantiation (4 waves x 32 x groups).
+int mfma_block_queries(int fmt, int dm);
+// Resident workgroups per CU of that instantiation (2 if it cannot be queried).
+int mfma_blocks_per_cu(int fmt, int dm, int cap);
 
 // ---------------------------------------------------------------------------
 // Low-precision preparation (knn_quant.hip)
@@ -116,6 +122,8 @@ template <class T>
 using LowdArgs = SearchArgs<T>;
 template <class T>
 bool launch_lowd_search(const LowdArgs<T>& a, hipStream_t s);
+template <class T>
+int lowd_blocks_per_cu(int d, int cap);
 
 // Merge the per-split candidate lists, re-rank them with exact FP64
 // direct-difference distances from the original rows, keep k.

@@ -27,16 +27,13 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   int qb, split;
-  {
+  {  // same XCD-aware order as the MFMA kernel
     const int b = blockIdx.x;
-    if ((a.n_splits & 7) == 0) {
-      const int xcd = b & 7, t = b >> 3;
-      split = xcd + 8 * (t / a.n_qblocks);
-      qb = t % a.n_qblocks;
-    } else {
-      split = b / a.n_qblocks;
-      qb = b % a.n_qblocks;
-    }
+    const int per_xcd = (int)(gridDim.x >> 3);
+    const int w_item = (b & 7) * per_xcd + (b >> 3);
+    if (w_item >= a.n_qblocks * a.n_splits) return;
+    split = w_item / a.n_qblocks;
+    qb = w_item % a.n_qblocks;
   }
   const int64_t q0 = (int64_t)qb * kThreads;
   const int64_t q = q0 + tid;
@@ -162,8 +159,19 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
 template <class T, int DP>
 void launch_dp(const LowdArgs<T>& a, hipStream_t s) {
   const size_t lds = (size_t)4 * a.cap * (sizeof(T) + sizeof(int));
-  hipLaunchKernelGGL((lowd_search_kernel<T, DP>), dim3((unsigned)(a.n_qblocks * a.n_splits)),
-                     dim3(kThreads), lds, s, a);
+  const unsigned blocks = (unsigned)((a.n_qblocks * a.n_splits + 7) / 8 * 8);
+  hipLaunchKernelGGL((lowd_search_kernel<T, DP>), dim3(blocks), dim3(kThreads), lds, s, a);
+}
+
+template <class T, int DP>
+int occupancy_dp(int cap) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lowd_search_kernel<T, DP>, kThreads,
+                                                   (size_t)4 * cap * (sizeof(T) + sizeof(int))) !=
+          hipSuccess ||
+      n <= 0)
+    return 2;
+  return n;
 }
 
 }  // namespace
@@ -177,6 +185,15 @@ bool launch_lowd_search(const LowdArgs<T>& a, hipStream_t s) {
   return true;
 }
 
+template <class T>
+int lowd_blocks_per_cu(int d, int cap) {
+  if (d <= 4) return occupancy_dp<T, 4>(cap);
+  if (d <= 8) return occupancy_dp<T, 8>(cap);
+  return occupancy_dp<T, 16>(cap);
+}
+
+template int lowd_blocks_per_cu<float>(int, int);
+template int lowd_blocks_per_cu<double>(int, int);
 template bool launch_lowd_search<float>(const LowdArgs<float>&, hipStream_t);
 template bool launch_lowd_search<double>(const LowdArgs<double>&, hipStream_t);
 

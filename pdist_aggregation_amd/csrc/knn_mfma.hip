@@ -84,27 +84,24 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   __shared__ __align__(16) unsigned char ytile[2][kBY * PITCH];
   __shared__ __align__(16) float yn_s[2][kBY];
   __shared__ int ye_s[2][kBY];  // FP8: E8M0 row scales
-  __shared__ int cnt_s[BQ];
+  __shared__ int cnt_s[2 * BQ];
   __shared__ float tau_s[BQ];
   extern __shared__ __align__(16) unsigned char dyn_smem[];  // 4 waves x cap x 8 B
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = lane >> 5, c = lane & 31;
 
-  // Workgroup -> (query block, split).  When the split count is a multiple of
-  // 8, all query blocks of one split land on one XCD (workgroups are dealt to
-  // the 8 XCDs round-robin) and run back to back, so they share that XCD's L2
-  // while streaming the same train rows.
+  // Workgroup -> (query block, split).  Workgroups are dealt to the 8 XCDs
+  // round-robin; work items are numbered split-major and each XCD takes a
+  // contiguous range of them, so the query blocks of one split run together
+  // on one XCD and share its L2 while they stream the same train rows.
   int qb, split;
   {
     const int b = blockIdx.x;
-    if ((a.n_splits & 7) == 0) {
-      const int xcd = b & 7, t = b >> 3;
-      split = xcd + 8 * (t / a.n_qblocks);
-      qb = t % a.n_qblocks;
-    } else {
-      split = b / a.n_qblocks;
-      qb = b % a.n_qblocks;
-    }
+    const int per_xcd = (int)(gridDim.x >> 3);
+    const int w_item = (b & 7) * per_xcd + (b >> 3);
+    if (w_item >= a.n_qblocks * a.n_splits) return;  // grid padding (whole workgroup)
+    split = w_item / a.n_qblocks;
+    qb = w_item % a.n_qblocks;
   }
   const int64_t q0 = (int64_t)qb * BQ;
   const int64_t ybeg = (int64_t)split * a.rows_per_split;
@@ -167,10 +164,21 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
       xsc[g] = qok ? (int)a.xscale[q] + 1 : 127;
     }
   }
-  for (int i = tid; i < BQ; i += kThreads) {
-    cnt_s[i] = 0;
-    tau_s[i] = FLT_MAX;
+  // Per-query candidate list = two halves of cap/2 entries; lane (c, h)
+  // appends to half h only, so its count lives in a register and appends
+  // need no atomics.  A step adds at most 16 entries per half.
+  const int half = a.cap >> 1;
+  const int thr = half - 16;
+  int cnt[QG];
+#pragma unroll
+  for (int g = 0; g < QG; ++g) {
+    cnt[g] = 0;
+    const int64_t q = q0 + wq0 + g * 32 + c;
+    if (q < a.n_q)  // start from the best threshold known (seeded or shared)
+      tau[g] = fminf(tau[g],
+                     (float)from_ordered_key(__atomic_load_n(&a.tau[q], __ATOMIC_RELAXED)));
   }
+  for (int i = tid; i < BQ; i += kThreads) tau_s[i] = FLT_MAX;
 
   // ---- tile loader: global -> registers -> LDS -------------------------------
   i32x4 pre[CPT];
@@ -210,31 +218,21 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     }
   };
 
-  load_tile(ybeg);
-  store_tile(0);
-  __syncthreads();
-
-  int cur = 0, step = 0;
-  for (int64_t y0 = ybeg; y0 < yend; y0 += kBY, ++step) {
-    const bool more = y0 + kBY < yend;
-    if (more) load_tile(y0 + kBY);
-
-    // C operand: |y|^2 of this lane's 16 rows (4 runs of 4 consecutive rows).
-    f32x16 acc[QG];
-    {
-      f32x16 yn;
+  // ---- one 32-row tile on the matrix cores ------------------------------------
+  // acc = |y|^2 (C operand of the first MFMA) - 2 x.y
+  auto compute = [&](f32x16(&acc)[QG], int buf) {
+    f32x16 yn;
 #pragma unroll
-      for (int r4 = 0; r4 < 4; ++r4) {
-        const f32x4 v = *reinterpret_cast<const f32x4*>(&yn_s[cur][r4 * 8 + h * 4]);
-        yn[r4 * 4 + 0] = v[0];
-        yn[r4 * 4 + 1] = v[1];
-        yn[r4 * 4 + 2] = v[2];
-        yn[r4 * 4 + 3] = v[3];
-      }
-#pragma unroll
-      for (int g = 0; g < QG; ++g) acc[g] = yn;
+    for (int r4 = 0; r4 < 4; ++r4) {
+      const f32x4 v = *reinterpret_cast<const f32x4*>(&yn_s[buf][r4 * 8 + h * 4]);
+      yn[r4 * 4 + 0] = v[0];
+      yn[r4 * 4 + 1] = v[1];
+      yn[r4 * 4 + 2] = v[2];
+      yn[r4 * 4 + 3] = v[3];
     }
-    const unsigned char* arow = &ytile[cur][c * PITCH + h * (DM / 2) * F::esize];
+#pragma unroll
+    for (int g = 0; g < QG; ++g) acc[g] = yn;
+    const unsigned char* arow = &ytile[buf][c * PITCH + h * (DM / 2) * F::esize];
     if constexpr (FMT == kFmtF32) {
 #pragma unroll
       for (int s = 0; s < NSTEP; s += 4) {
@@ -256,7 +254,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     } else {
       // Row scales ride in the MFMA (E8M0 per 32-element block): A's block is
       // one train row, B's one query, so D is already |y|^2 - 2 x.y.
-      const int sa = ye_s[cur][c];
+      const int sa = ye_s[buf][c];
 #pragma unroll
       for (int s = 0; s < NSTEP; ++s) {
         const i32x4 lo = *reinterpret_cast<const i32x4*>(arow + s * 32);
@@ -268,8 +266,84 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
                                                                    sa, 0, xsc[g]);
       }
     }
+  };
 
-    // ---- selection ----------------------------------------------------------
+  // ---- list maintenance (wave-local) ----------------------------------------
+  // Sort the two halves of query ql together, keep the best min(n, kc).
+  auto sort_query = [&](int ql, int n0, int n1) {
+    float* cd = cand_d + (size_t)ql * a.cap;
+    int* ci = cand_i + (size_t)ql * a.cap;
+    const int n = n0 + n1;
+    const int P = next_pow2(n);
+    for (int e = lane; e < P; e += 64) {
+      const bool ok = e < n;
+      const int src = e < n0 ? e : half + (e - n0);
+      sd[e] = ok ? cd[src] : __builtin_huge_valf();
+      si[e] = ok ? ci[src] : INT_MAX;
+    }
+    wave_sync();
+    for (int size = 2; size <= P; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int i = lane; i < (P >> 1); i += 64) {
+          const int lo = 2 * stride * (i / stride) + (i % stride);
+          const int hi = lo + stride;
+          const bool up = (lo & size) == 0;
+          const float dl = sd[lo], dh = sd[hi];
+          const int il = si[lo], ih = si[hi];
+          const bool swap = up ? key_less(dh, ih, dl, il) : key_less(dl, il, dh, ih);
+          if (swap) {
+            sd[lo] = dh;
+            sd[hi] = dl;
+            si[lo] = ih;
+            si[hi] = il;
+          }
+        }
+        wave_sync();
+      }
+    }
+  };
+  // Compact every query of this wave whose half exceeds `limit`.
+  auto compact = [&](int limit) {
+#pragma unroll
+    for (int g = 0; g < QG; ++g) cnt_s[2 * (wq0 + g * 32 + c) + h] = cnt[g];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    wave_sync();
+    for (int qq = 0; qq < WQ; ++qq) {
+      const int ql = wq0 + qq;
+      const int n0 = cnt_s[2 * ql], n1 = cnt_s[2 * ql + 1];
+      if (n0 <= limit && n1 <= limit) continue;
+      sort_query(ql, n0, n1);
+      const int m = n0 + n1 < a.kc ? n0 + n1 : a.kc;
+      const int m0 = (m + 1) >> 1;
+      float* cd = cand_d + (size_t)ql * a.cap;
+      int* ci = cand_i + (size_t)ql * a.cap;
+      for (int e = lane; e < m; e += 64) {
+        const int dst = e < m0 ? e : half + (e - m0);
+        cd[dst] = sd[e];
+        ci[dst] = si[e];
+      }
+      if (lane == 0) {
+        cnt_s[2 * ql] = m0;
+        cnt_s[2 * ql + 1] = m - m0;
+        if (m == a.kc) {
+          const float kth = sd[a.kc - 1];
+          if (kth < tau_s[ql]) tau_s[ql] = kth;
+          atomicMin(&a.tau[q0 + ql], ordered_key((double)kth));
+        }
+      }
+      wave_sync();
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+#pragma unroll
+    for (int g = 0; g < QG; ++g) {
+      const int ql = wq0 + g * 32 + c;
+      cnt[g] = cnt_s[2 * ql + h];
+      tau[g] = fminf(tau[g], tau_s[ql]);
+    }
+  };
+  // Selection of one finished tile: min-tree + one compare per lane in the
+  // common case; survivors appended to the lane's half-list.
+  auto select = [&](f32x16(&acc)[QG], int64_t y0, int stp) {
     bool overflow = false;
 #pragma unroll
     for (int g = 0; g < QG; ++g) {
@@ -278,43 +352,24 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
       for (int i = 1; i < 16; ++i) m = fminf(m, acc[g][i]);
       if (__any(m <= tau[g])) {
         const int ql = wq0 + g * 32 + c;
-        float* cd = cand_d + (size_t)ql * a.cap;
-        int* ci = cand_i + (size_t)ql * a.cap;
+        float* cd = cand_d + (size_t)ql * a.cap + h * half;
+        int* ci = cand_i + (size_t)ql * a.cap + h * half;
+        int n = cnt[g];
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const float v = acc[g][i];
           if (v <= tau[g]) {
-            const int pos = atomicAdd(&cnt_s[ql], 1);
-            cd[pos] = v;
-            ci[pos] = (int)(y0 + (i >> 2) * 8 + h * 4 + (i & 3));
-            overflow |= pos + 1 > a.cap - kBY;
+            cd[n] = v;
+            ci[n] = (int)(y0 + (i >> 2) * 8 + h * 4 + (i & 3));
+            ++n;
           }
         }
+        cnt[g] = n;
+        overflow |= n > thr;
       }
     }
-    if (__any(overflow)) {
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      for (int qq = 0; qq < WQ; ++qq) {
-        const int ql = wq0 + qq;
-        const int n = cnt_s[ql];
-        if (n <= a.cap - kBY) continue;
-        float* cd = cand_d + (size_t)ql * a.cap;
-        int* ci = cand_i + (size_t)ql * a.cap;
-        wave_sort_cut(cd, ci, n, a.kc, sd, si, lane);
-        if (lane == 0) {
-          cnt_s[ql] = n < a.kc ? n : a.kc;
-          if (n >= a.kc) {
-            const float kth = sd[a.kc - 1];
-            if (kth < tau_s[ql]) tau_s[ql] = kth;
-            atomicMin(&a.tau[q0 + ql], ordered_key((double)kth));
-          }
-        }
-        wave_sync();
-      }
-#pragma unroll
-      for (int g = 0; g < QG; ++g) tau[g] = fminf(tau[g], tau_s[wq0 + g * 32 + c]);
-    }
-    if ((step & 15) == 15) {
+    if (__any(overflow)) compact(thr);
+    if ((stp & 15) == 15) {
 #pragma unroll
       for (int g = 0; g < QG; ++g) {
         const int64_t q = q0 + wq0 + g * 32 + c;
@@ -325,74 +380,132 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
         }
       }
     }
+  };
 
-    if (more) store_tile(cur ^ 1);
+  // ---- main sweep: the MFMAs of tile t are issued before the selection of
+  // tile t-1, so the selection's VALU work overlaps the matrix cores. ---------
+  load_tile(ybeg);
+  store_tile(0);
+  __syncthreads();
+  int cur = 0, step = 0;
+  f32x16 accA[QG], accB[QG];
+  auto advance = [&](int64_t y0) {
+    if (y0 + kBY < yend) store_tile(cur ^ 1);
     __syncthreads();
     cur ^= 1;
+  };
+  int64_t y0 = ybeg;
+  if (y0 < yend) {
+    if (y0 + kBY < yend) load_tile(y0 + kBY);
+    compute(accA, cur);
+    advance(y0);
+    y0 += kBY;
+    for (;;) {
+      if (y0 >= yend) {
+        select(accA, y0 - kBY, step++);
+        break;
+      }
+      if (y0 + kBY < yend) load_tile(y0 + kBY);
+      compute(accB, cur);
+      select(accA, y0 - kBY, step++);
+      advance(y0);
+      y0 += kBY;
+      if (y0 >= yend) {
+        select(accB, y0 - kBY, step++);
+        break;
+      }
+      if (y0 + kBY < yend) load_tile(y0 + kBY);
+      compute(accA, cur);
+      select(accB, y0 - kBY, step++);
+      advance(y0);
+      y0 += kBY;
+    }
   }
 
-  // ---- final: sort and cut every list of this wave, write the split slot ----
+  // ---- final: sort every list of this wave, write the split's slot --------
+#pragma unroll
+  for (int g = 0; g < QG; ++g) cnt_s[2 * (wq0 + g * 32 + c) + h] = cnt[g];
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-  for (int qq = 0; qq < WQ; ++qq) {
-    const int ql = wq0 + qq;
-    const int n = cnt_s[ql];
-    if (n <= 0) continue;
-    wave_sort_cut(cand_d + (size_t)ql * a.cap, cand_i + (size_t)ql * a.cap, n, a.kc, sd, si,
-                  lane);
-    if (lane == 0) cnt_s[ql] = n < a.kc ? n : a.kc;
-    wave_sync();
-  }
+  wave_sync();
   for (int qq = 0; qq < WQ; ++qq) {
     const int ql = wq0 + qq;
     const int64_t q = q0 + ql;
     if (q >= a.n_q) break;
-    const int n = cnt_s[ql];
+    const int n0 = cnt_s[2 * ql], n1 = cnt_s[2 * ql + 1];
+    const int n = n0 + n1;
+    if (n > 0) sort_query(ql, n0, n1);
+    const int m = n < a.kc ? n : a.kc;
     for (int j = lane; j < a.kc; j += 64) {
       const size_t o = ((size_t)q * a.n_splits + split) * a.kc + j;
-      a.part_d[o] = j < n ? cand_d[(size_t)ql * a.cap + j] : __builtin_huge_valf();
-      a.part_i[o] = j < n ? cand_i[(size_t)ql * a.cap + j] : -1;
+      a.part_d[o] = j < m ? sd[j] : __builtin_huge_valf();
+      a.part_i[o] = j < m ? si[j] : -1;
     }
+    wave_sync();
   }
 }
 
 template <int FMT, int DM, int QG>
 void launch_one(const MfmaArgs& a, hipStream_t s) {
   const size_t lds = (size_t)4 * a.cap * 8;
-  hipLaunchKernelGGL((mfma_search_kernel<FMT, DM, QG>), dim3((unsigned)(a.n_qblocks * a.n_splits)),
-                     dim3(kThreads), lds, s, a);
+  const unsigned blocks = (unsigned)((a.n_qblocks * a.n_splits + 7) / 8 * 8);
+  hipLaunchKernelGGL((mfma_search_kernel<FMT, DM, QG>), dim3(blocks), dim3(kThreads), lds, s, a);
+}
+
+template <int FMT, int DM, int QG>
+int occupancy_one(int cap) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mfma_search_kernel<FMT, DM, QG>, kThreads,
+                                                   (size_t)4 * cap * 8) != hipSuccess || n <= 0)
+    return 2;
+  return n;
 }
 
 }  // namespace
 
-static_assert(kMfmaQueriesPerBlock == 4 * 2 * 32, "instantiations use 2 query groups per wave");
 static_assert(kMfmaRowStep == kBY, "row step");
 
+// Query groups (of 32) per wave: two where the resident queries fit in the
+// register budget of 2 waves/SIMD next to two accumulator sets, else one.
+constexpr int groups_for(int fmt, int dm) {
+  return ((fmt == kFmtF32 && dm >= 128) || dm >= 256) ? 1 : 2;
+}
+
+int mfma_block_queries(int fmt, int dm) { return 4 * 32 * groups_for(fmt, dm); }
+
+int mfma_blocks_per_cu(int fmt, int dm, int cap) {
+#define KNN_OCC_CASE(F, D) \
+  if (fmt == F && dm == D) return occupancy_one<F, D, groups_for(F, D)>(cap);
+  KNN_OCC_CASE(kFmtF32, 32)
+  KNN_OCC_CASE(kFmtF32, 64)
+  KNN_OCC_CASE(kFmtF32, 128)
+  KNN_OCC_CASE(kFmtBF16, 32)
+  KNN_OCC_CASE(kFmtBF16, 64)
+  KNN_OCC_CASE(kFmtBF16, 128)
+  KNN_OCC_CASE(kFmtBF16, 256)
+  KNN_OCC_CASE(kFmtFP8, 64)
+  KNN_OCC_CASE(kFmtFP8, 128)
+  KNN_OCC_CASE(kFmtFP8, 256)
+#undef KNN_OCC_CASE
+  return 2;
+}
+
 bool launch_mfma_search(const MfmaArgs& a, hipStream_t s) {
-  if (a.fmt == kFmtF32) {
-    switch (a.dm) {
-      case 32: launch_one<kFmtF32, 32, 2>(a, s); return true;
-      case 64: launch_one<kFmtF32, 64, 2>(a, s); return true;
-      case 128: launch_one<kFmtF32, 128, 2>(a, s); return true;
-      default: return false;
-    }
+#define KNN_MFMA_CASE(F, D)                         \
+  if (a.fmt == F && a.dm == D) {                    \
+    launch_one<F, D, groups_for(F, D)>(a, s);       \
+    return true;                                    \
   }
-  if (a.fmt == kFmtFP8) {
-    switch (a.dm) {
-      case 64: launch_one<kFmtFP8, 64, 2>(a, s); return true;
-      case 128: launch_one<kFmtFP8, 128, 2>(a, s); return true;
-      case 256: launch_one<kFmtFP8, 256, 2>(a, s); return true;
-      default: return false;
-    }
-  }
-  if (a.fmt == kFmtBF16) {
-    switch (a.dm) {
-      case 32: launch_one<kFmtBF16, 32, 2>(a, s); return true;
-      case 64: launch_one<kFmtBF16, 64, 2>(a, s); return true;
-      case 128: launch_one<kFmtBF16, 128, 2>(a, s); return true;
-      case 256: launch_one<kFmtBF16, 256, 2>(a, s); return true;
-      default: return false;
-    }
-  }
+  KNN_MFMA_CASE(kFmtF32, 32)
+  KNN_MFMA_CASE(kFmtF32, 64)
+  KNN_MFMA_CASE(kFmtF32, 128)
+  KNN_MFMA_CASE(kFmtBF16, 32)
+  KNN_MFMA_CASE(kFmtBF16, 64)
+  KNN_MFMA_CASE(kFmtBF16, 128)
+  KNN_MFMA_CASE(kFmtBF16, 256)
+  KNN_MFMA_CASE(kFmtFP8, 64)
+  KNN_MFMA_CASE(kFmtFP8, 128)
+  KNN_MFMA_CASE(kFmtFP8, 256)
+#undef KNN_MFMA_CASE
   return false;
 }
 
