@@ -73,7 +73,7 @@ struct Plan {
   // workspace offsets
   size_t off_scalars = 0, off_ynorm = 0, off_tau = 0, off_cand_d = 0, off_cand_i = 0,
          off_part_d = 0, off_part_i = 0, off_mu = 0, off_xq = 0, off_yq = 0, off_xnorm = 0,
-         off_xerr = 0, off_xscale = 0, off_yscale = 0, total = 0;
+         off_xerr = 0, off_xscale = 0, off_yscale = 0, off_slots = 0, total = 0;
 };
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
@@ -232,6 +232,8 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   const size_t npart = (size_t)nq1 * p.n_splits * p.kc;
   p.off_part_d = take(npart * celem);
   p.off_part_i = take(npart * sizeof(int));
+  if (p.path == kPathMfmaF32 || p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8)
+    p.off_slots = take((size_t)nq1 * p.n_splits * sizeof(float) + 8);
   if (p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8) {
     const size_t qe = p.path == kPathMfmaBF16 ? 2 : 1;
     p.off_mu = take(2 * (size_t)p.dm * sizeof(float));
@@ -377,6 +379,17 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     fa.part_d = reinterpret_cast<float*>(ws + p.off_part_d);
     fa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
     fa.ynorm = reinterpret_cast<float*>(ynorm);
+    // Shared-threshold slots (+inf = nothing published), used when at least
+    // kSlotJ splits exist.
+    fa.qslots = nullptr;
+    fa.slot_rank = (int)knn::ceil_div(p.kc, knn::kSlotJ) - 1;
+    if (p.n_splits >= knn::kSlotJ) {
+      fa.qslots = reinterpret_cast<float*>(ws + p.off_slots);
+      const unsigned long long two_inf = 0x7f8000007f800000ull;
+      knn::launch_fill_keys<float>(reinterpret_cast<unsigned long long*>(fa.qslots),
+                                   knn::ceil_div((int64_t)n_test * p.n_splits, 2), two_inf,
+                                   stream);
+    }
     if (p.path == kPathMfmaF32) {
       // 2. norms
       knn::launch_row_norms<float>(reinterpret_cast<const float*>(X_train), n_train, (int)dim,

@@ -78,7 +78,14 @@ struct MfmaArgs {
   unsigned long long* tau;
   float* part_d;
   int* part_i;
+  // Cross-split threshold sharing: qslots[q][split] (init +inf) receives the
+  // split's (slot_rank+1)-th smallest value at each list compaction, with
+  // kSlotJ * (slot_rank+1) >= k'; nullptr disables it.
+  float* qslots;
+  int slot_rank;
 };
+
+constexpr int kSlotJ = 4;
 
 // Returns false when no instantiation matches (fmt, dm).
 bool launch_mfma_search(const MfmaArgs& a, hipStream_t s);

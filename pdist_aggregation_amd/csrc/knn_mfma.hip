@@ -330,6 +330,11 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
           if (kth < tau_s[ql]) tau_s[ql] = kth;
           atomicMin(&a.tau[q0 + ql], ordered_key((double)kth));
         }
+        // Publish this split's (slot_rank+1)-th smallest value (see select()).
+        if (a.qslots && m > a.slot_rank)
+          __atomic_store_n(reinterpret_cast<unsigned*>(a.qslots) + (size_t)(q0 + ql) * a.n_splits +
+                               split,
+                           __float_as_uint(sd[a.slot_rank]), __ATOMIC_RELAXED);
       }
       wave_sync();
     }
@@ -369,13 +374,31 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
       }
     }
     if (__any(overflow)) compact(thr);
-    if ((stp & 15) == 15) {
+    if ((stp & 7) == 7) {
+      // Thresholds shared across the splits of a query:
+      //  * the best k'-th value any split has reached (atomicMin key);
+      //  * slot s holds split s's (slot_rank+1)-th smallest value, so the
+      //    kSlotJ-th smallest slot value T has at least
+      //    kSlotJ * (slot_rank+1) >= k' rows with v <= T: T bounds the
+      //    global k'-th value.  Sorted insertion keeps the kSlotJ smallest.
 #pragma unroll
       for (int g = 0; g < QG; ++g) {
         const int64_t q = q0 + wq0 + g * 32 + c;
         if (q < a.n_q) {
-          const float t =
-              (float)from_ordered_key(__atomic_load_n(&a.tau[q], __ATOMIC_RELAXED));
+          float t = (float)from_ordered_key(__atomic_load_n(&a.tau[q], __ATOMIC_RELAXED));
+          if (a.qslots) {
+            float r[kSlotJ];
+#pragma unroll
+            for (int j = 0; j < kSlotJ; ++j) r[j] = __builtin_huge_valf();
+            const unsigned* sl = reinterpret_cast<const unsigned*>(a.qslots) + (size_t)q * a.n_splits;
+            for (int sidx = 0; sidx < a.n_splits; ++sidx) {
+              const float x = __uint_as_float(__atomic_load_n(sl + sidx, __ATOMIC_RELAXED));
+#pragma unroll
+              for (int j = kSlotJ - 1; j > 0; --j) r[j] = fminf(r[j], fmaxf(r[j - 1], x));
+              r[0] = fminf(r[0], x);
+            }
+            t = fminf(t, r[kSlotJ - 1]);
+          }
           tau[g] = fminf(tau[g], t);
         }
       }
