@@ -211,6 +211,14 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
                                        p.dm, p.cap);
     splits = choose_splits_slots(p.n_qblocks, n_train, step, p.kc,
                                  (int64_t)device_cus() * per_cu);
+    // The MFMA kernel addresses one split with 32-bit buffer offsets.
+    if (p.path != kPathLowd) {
+      const int64_t row_bytes = p.path == kPathMfmaF32 ? dim * 4
+                                : p.path == kPathMfmaBF16 ? (int64_t)p.dm * 2
+                                                          : (int64_t)p.dm;
+      const int64_t min_splits = knn::ceil_div(n_train * row_bytes, (int64_t)1 << 30);
+      splits = (int)std::max<int64_t>(splits, min_splits);
+    }
   }
   p.rows_per_split = knn::ceil_div(knn::ceil_div(n_train, splits), step) * step;
   p.n_splits = (int)knn::ceil_div(n_train, p.rows_per_split);

@@ -181,41 +181,62 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   for (int i = tid; i < BQ; i += kThreads) tau_s[i] = FLT_MAX;
 
   // ---- tile loader: global -> registers -> LDS -------------------------------
-  i32x4 pre[CPT];
-  float pre_n = 0.f;
-  int pre_e = 127;
-  auto load_tile = [&](int64_t y0) {
-#pragma unroll
-    for (int i = 0; i < CPT; ++i) {
-      const int ch = tid + i * kThreads;
-      const int row = ch / (DM * F::esize / 16);
-      const int col = ch % (DM * F::esize / 16);
-      const int64_t y = y0 + row;
-      const int f = col * (16 / F::esize);
-      i32x4 v = {0, 0, 0, 0};
-      if (ch < CHUNKS && y < yend && f < a.d)
-        v = *reinterpret_cast<const i32x4*>(reinterpret_cast<const unsigned char*>(a.Y) +
-                                           ((size_t)y * a.ldy + f) * F::esize);
-      pre[i] = v;
-    }
-    if (tid < kBY) {
-      const int64_t y = y0 + tid;
-      pre_n = y < yend ? a.ynorm[y] : __builtin_huge_valf();
-      if constexpr (FMT == kFmtFP8) pre_e = y < yend ? (int)a.yscale[y] : 127;
-    }
+  // Two register sets: the tile two steps ahead is in flight while the next
+  // one is written to LDS.
+  struct Staged {
+    i32x4 v[CPT];
+    float n;
+    int e;
+    int64_t t;
   };
-  auto store_tile = [&](int buf) {
+  // Raw buffer loads over this split's rows: rows past the split read as
+  // zero (range-checked by the hardware), so the loads carry no branches and
+  // the compiler can wait for exactly the loads it needs.
+  const size_t row_bytes = (size_t)a.ldy * F::esize;
+  const int64_t nrows = yend - ybeg;
+  const __amdgpu_buffer_rsrc_t yrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<unsigned char*>(reinterpret_cast<const unsigned char*>(a.Y)) + ybeg * row_bytes,
+      (short)0, (int)(nrows * row_bytes), 0x00020000);
+  const __amdgpu_buffer_rsrc_t nrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(a.ynorm + ybeg), (short)0, (int)(nrows * 4), 0x00020000);
+  __amdgpu_buffer_rsrc_t ers = nrs;
+  if constexpr (FMT == kFmtFP8)
+    ers = __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned char*>(a.yscale + ybeg), (short)0,
+                                            (int)nrows, 0x00020000);
+  auto load_tile = [&](Staged& st, int64_t t) {  // tile t of this split
 #pragma unroll
     for (int i = 0; i < CPT; ++i) {
       const int ch = tid + i * kThreads;
       const int row = ch / (DM * F::esize / 16);
       const int col = ch % (DM * F::esize / 16);
-      if (ch < CHUNKS) *reinterpret_cast<i32x4*>(&ytile[buf][row * PITCH + col * 16]) = pre[i];
+      const int f = col * (16 / F::esize);
+      const int off = (int)((t * kBY + row) * row_bytes) + f * F::esize;
+      st.v[i] = __builtin_amdgcn_raw_buffer_load_b128(yrs, off, 0, 0);
     }
-    if (tid < kBY) {
-      yn_s[buf][tid] = pre_n;
-      ye_s[buf][tid] = pre_e;
+    const int r = (int)(t * kBY) + (tid & (kBY - 1));
+    st.n = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(nrs, r * 4, 0, 0));
+    st.e = 127;
+    if constexpr (FMT == kFmtFP8) st.e = (int)__builtin_amdgcn_raw_buffer_load_b8(ers, r, 0, 0);
+    st.t = t;
+  };
+  // Masking happens here, not at the load, so nothing waits on the data
+  // before it is needed: features past d read as 0, rows past the split get
+  // |y|^2 = +inf (never selected).
+  auto store_tile = [&](const Staged& st, int buf) {
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const int ch = tid + i * kThreads;
+      const int row = ch / (DM * F::esize / 16);
+      const int col = ch % (DM * F::esize / 16);
+      const int f = col * (16 / F::esize);
+      const i32x4 v = f < a.d ? st.v[i] : i32x4{0, 0, 0, 0};
+      if (ch < CHUNKS) *reinterpret_cast<i32x4*>(&ytile[buf][row * PITCH + col * 16]) = v;
     }
+    // Every thread holds the norm of row (tid % 32); the 8 copies write the
+    // same value, without a branch.
+    const int r = (int)(st.t * kBY) + (tid & (kBY - 1));
+    yn_s[buf][tid & (kBY - 1)] = r < nrows ? st.n : __builtin_huge_valf();
+    if constexpr (FMT == kFmtFP8) ye_s[buf][tid & (kBY - 1)] = r < nrows ? st.e : 127;
   };
 
   // ---- one 32-row tile on the matrix cores ------------------------------------
@@ -405,44 +426,40 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     }
   };
 
-  // ---- main sweep: the MFMAs of tile t are issued before the selection of
-  // tile t-1, so the selection's VALU work overlaps the matrix cores. ---------
-  load_tile(ybeg);
-  store_tile(0);
-  __syncthreads();
-  int cur = 0, step = 0;
+  // ---- main sweep --------------------------------------------------------
+  // Step t (tile t in LDS buffer t%2, tile t+1 in flight in registers):
+  //   issue the loads of tile t+2; MFMAs of tile t; write tile t+1 to LDS;
+  //   select tile t-1 (its VALU work overlaps the MFMAs); barrier.
+  // Writing tile t+1 waits for every older vector-memory operation, which
+  // are then a full step old: the loads of tile t+1 and the candidate stores
+  // of the previous step's selection.
+  const int64_t ntiles = yend > ybeg ? (yend - ybeg + kBY - 1) / kBY : 0;
+  Staged stA, stB;
   f32x16 accA[QG], accB[QG];
-  auto advance = [&](int64_t y0) {
-    if (y0 + kBY < yend) store_tile(cur ^ 1);
+  load_tile(stA, 0);
+  store_tile(stA, 0);
+  if (ntiles > 1) load_tile(stB, 1);
+  __syncthreads();
+  // One step; `cur` is the LDS buffer of tile t, `ld` the free register set,
+  // `st` the set holding tile t+1.
+  auto one_step = [&](int64_t t, f32x16(&accC)[QG], f32x16(&accS)[QG], Staged& ld,
+                      Staged& st) {
+    const int cur = (int)(t & 1);
+    if (t + 2 < ntiles) load_tile(ld, t + 2);
+    compute(accC, cur);
+    if (t + 1 < ntiles) store_tile(st, cur ^ 1);
+    if (t > 0) select(accS, ybeg + (t - 1) * kBY, (int)(t - 1));
     __syncthreads();
-    cur ^= 1;
   };
-  int64_t y0 = ybeg;
-  if (y0 < yend) {
-    if (y0 + kBY < yend) load_tile(y0 + kBY);
-    compute(accA, cur);
-    advance(y0);
-    y0 += kBY;
-    for (;;) {
-      if (y0 >= yend) {
-        select(accA, y0 - kBY, step++);
-        break;
-      }
-      if (y0 + kBY < yend) load_tile(y0 + kBY);
-      compute(accB, cur);
-      select(accA, y0 - kBY, step++);
-      advance(y0);
-      y0 += kBY;
-      if (y0 >= yend) {
-        select(accB, y0 - kBY, step++);
-        break;
-      }
-      if (y0 + kBY < yend) load_tile(y0 + kBY);
-      compute(accA, cur);
-      select(accB, y0 - kBY, step++);
-      advance(y0);
-      y0 += kBY;
-    }
+  for (int64_t t = 0; t < ntiles; t += 2) {
+    one_step(t, accA, accB, stA, stB);
+    if (t + 1 < ntiles) one_step(t + 1, accB, accA, stB, stA);
+  }
+  if (ntiles > 0) {
+    if ((ntiles - 1) & 1)
+      select(accB, ybeg + (ntiles - 1) * kBY, (int)(ntiles - 1));
+    else
+      select(accA, ybeg + (ntiles - 1) * kBY, (int)(ntiles - 1));
   }
 
   // ---- final: sort every list of this wave, write the split's slot --------
