@@ -69,7 +69,7 @@ __device__ void block_bitonic_sort(T* sd, int* si, int n) {
   for (int size = 2; size <= n; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       for (int i = threadIdx.x; i < (n >> 1); i += blockDim.x) {
-        const int lo = 2 * stride * (i / stride) + (i % stride);
+        const int lo = ((i & ~(stride - 1)) << 1) | (i & (stride - 1));  // stride: power of 2
         const int hi = lo + stride;
         const bool up = (lo & size) == 0;
         const T dl = sd[lo], dh = sd[hi];
@@ -85,6 +85,16 @@ __device__ void block_bitonic_sort(T* sd, int* si, int n) {
       __syncthreads();
     }
   }
+}
+
+// Relaxed loads at device (agent) scope: coherent in L2 without going to
+// memory, for values other workgroups update (shared thresholds).
+__device__ inline unsigned long long load_agent(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float load_agent(const float* p) {
+  return __uint_as_float(__hip_atomic_load(reinterpret_cast<const unsigned*>(p), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT));
 }
 
 // Orders LDS (and same-CU global) accesses between the lanes of one wave.
@@ -112,7 +122,7 @@ __device__ void wave_sort_cut(T* cd, int* ci, int n, int kc, T* sd, int* si, int
   for (int size = 2; size <= P; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       for (int i = lane; i < (P >> 1); i += 64) {
-        const int lo = 2 * stride * (i / stride) + (i % stride);
+        const int lo = ((i & ~(stride - 1)) << 1) | (i & (stride - 1));  // stride: power of 2
         const int hi = lo + stride;
         const bool up = (lo & size) == 0;
         const T dl = sd[lo], dh = sd[hi];

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
       if (tid == 0) flag_s = 0;
     }
     if ((step & 7) == 7 && tid < n_q_here) {
-      const T g = (T)from_ordered_key(__atomic_load_n(&a.tau[q0 + tid], __ATOMIC_RELAXED));
+      const T g = (T)from_ordered_key(load_agent(&a.tau[q0 + tid]));
       if (g < tau_s[tid]) tau_s[tid] = g;
     }
     __syncthreads();

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
       }
     }
     if ((step & 15) == 15 && qok) {
-      const T g = (T)from_ordered_key(__atomic_load_n(&a.tau[q], __ATOMIC_RELAXED));
+      const T g = (T)from_ordered_key(load_agent(&a.tau[q]));
       tau = g < tau ? g : tau;
     }
     if (more) store_tile(cur ^ 1);

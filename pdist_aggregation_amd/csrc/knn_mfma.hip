@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     const int64_t q = q0 + wq0 + g * 32 + c;
     if (q < a.n_q)  // start from the best threshold known (seeded or shared)
       tau[g] = fminf(tau[g],
-                     (float)from_ordered_key(__atomic_load_n(&a.tau[q], __ATOMIC_RELAXED)));
+                     (float)from_ordered_key(load_agent(&a.tau[q])));
   }
   for (int i = tid; i < BQ; i += kThreads) tau_s[i] = FLT_MAX;
 
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     for (int size = 2; size <= P; size <<= 1) {
       for (int stride = size >> 1; stride > 0; stride >>= 1) {
         for (int i = lane; i < (P >> 1); i += 64) {
-          const int lo = 2 * stride * (i / stride) + (i % stride);
+          const int lo = ((i & ~(stride - 1)) << 1) | (i & (stride - 1));  // stride: power of 2
           const int hi = lo + stride;
           const bool up = (lo & size) == 0;
           const float dl = sd[lo], dh = sd[hi];
@@ -395,7 +395,8 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
       }
     }
     if (__any(overflow)) compact(thr);
-    if ((stp & 7) == 7) {
+    // Refresh at steps 3, 7, 15, then every 32 (cheap early sharing, rare later).
+    if ((stp < 32 && ((stp + 1) & stp) == 0 && stp >= 3) || (stp & 31) == 31) {
       // Thresholds shared across the splits of a query:
       //  * the best k'-th value any split has reached (atomicMin key);
       //  * slot s holds split s's (slot_rank+1)-th smallest value, so the
@@ -406,19 +407,22 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
       for (int g = 0; g < QG; ++g) {
         const int64_t q = q0 + wq0 + g * 32 + c;
         if (q < a.n_q) {
-          float t = (float)from_ordered_key(__atomic_load_n(&a.tau[q], __ATOMIC_RELAXED));
+          float t = (float)from_ordered_key(load_agent(&a.tau[q]));
           if (a.qslots) {
             float r[kSlotJ];
 #pragma unroll
             for (int j = 0; j < kSlotJ; ++j) r[j] = __builtin_huge_valf();
-            const unsigned* sl = reinterpret_cast<const unsigned*>(a.qslots) + (size_t)q * a.n_splits;
+            const float* sl = a.qslots + (size_t)q * a.n_splits;
             for (int sidx = 0; sidx < a.n_splits; ++sidx) {
-              const float x = __uint_as_float(__atomic_load_n(sl + sidx, __ATOMIC_RELAXED));
+              const float x = load_agent(sl + sidx);
 #pragma unroll
-              for (int j = kSlotJ - 1; j > 0; --j) r[j] = fminf(r[j], fmaxf(r[j - 1], x));
-              r[0] = fminf(r[0], x);
+              for (int j = kSlotJ - 1; j > 0; --j) {
+                const float hi = r[j - 1] > x ? r[j - 1] : x;
+                r[j] = r[j] < hi ? r[j] : hi;
+              }
+              r[0] = r[0] < x ? r[0] : x;
             }
-            t = fminf(t, r[kSlotJ - 1]);
+            t = t < r[kSlotJ - 1] ? t : r[kSlotJ - 1];
           }
           tau[g] = fminf(tau[g], t);
         }
