@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   const T* pd = a.part_d + (size_t)q * total;
   const int* pi = a.part_i + (size_t)q * total;
   for (int e = tid; e < P; e += kThreads) {
-    const bool ok = e < total && pi[e] >= 0;
+    const bool ok = e < total && pi[e] >= 0 && pi[e] < a.n_t;  // never index past Y
     sd[e] = ok ? pd[e] : pos_inf<T>();
     si[e] = ok ? pi[e] : INT_MAX;
   }
