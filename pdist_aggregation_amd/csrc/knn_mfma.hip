@@ -229,8 +229,11 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
       const int row = ch / (DM * F::esize / 16);
       const int col = ch % (DM * F::esize / 16);
       const int f = col * (16 / F::esize);
-      const i32x4 v = f < a.d ? st.v[i] : i32x4{0, 0, 0, 0};
-      if (ch < CHUNKS) *reinterpret_cast<i32x4*>(&ytile[buf][row * PITCH + col * 16]) = v;
+      // BF16/FP8 rows are zero-padded to DM by the quantiser; only float32
+      // rows of d < DM features need the mask.
+      const i32x4 v = (FMT != kFmtF32 || f < a.d) ? st.v[i] : i32x4{0, 0, 0, 0};
+      if (CHUNKS % kThreads == 0 || ch < CHUNKS)
+        *reinterpret_cast<i32x4*>(&ytile[buf][row * PITCH + col * 16]) = v;
     }
     // Every thread holds the norm of row (tid % 32); the 8 copies write the
     // same value, without a branch.
@@ -367,27 +370,45 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
       tau[g] = fminf(tau[g], tau_s[ql]);
     }
   };
-  // Selection of one finished tile: min-tree + one compare per lane in the
-  // common case; survivors appended to the lane's half-list.
+  // Appends go through buffer stores on this workgroup's candidate block, so
+  // the address of an append is one 32-bit offset.
+  const int cand_bytes = BQ * a.cap * 4;
+  const __amdgpu_buffer_rsrc_t cdrs =
+      __builtin_amdgcn_make_buffer_rsrc(cand_d, (short)0, cand_bytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t cirs =
+      __builtin_amdgcn_make_buffer_rsrc(cand_i, (short)0, cand_bytes, 0x00020000);
+  // Selection of one finished tile.  Lane values come in 4 quads of 4
+  // consecutive train rows; the common case costs a min-tree and one
+  // compare, a tile with survivors tests the 4 quad minima and then only the
+  // values of quads that hold a survivor.
   auto select = [&](f32x16(&acc)[QG], int64_t y0, int stp) {
     bool overflow = false;
 #pragma unroll
     for (int g = 0; g < QG; ++g) {
-      float m = acc[g][0];
+      float m4[4];
 #pragma unroll
-      for (int i = 1; i < 16; ++i) m = fminf(m, acc[g][i]);
+      for (int r4 = 0; r4 < 4; ++r4)
+        m4[r4] = fminf(fminf(acc[g][4 * r4], acc[g][4 * r4 + 1]),
+                       fminf(acc[g][4 * r4 + 2], acc[g][4 * r4 + 3]));
+      const float m = fminf(fminf(m4[0], m4[1]), fminf(m4[2], m4[3]));
       if (__any(m <= tau[g])) {
         const int ql = wq0 + g * 32 + c;
-        float* cd = cand_d + (size_t)ql * a.cap + h * half;
-        int* ci = cand_i + (size_t)ql * a.cap + h * half;
+        const int base = (ql * a.cap + h * half) * 4;
+        const int yb = (int)y0 + h * 4;
         int n = cnt[g];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const float v = acc[g][i];
-          if (v <= tau[g]) {
-            cd[n] = v;
-            ci[n] = (int)(y0 + (i >> 2) * 8 + h * 4 + (i & 3));
-            ++n;
+        for (int r4 = 0; r4 < 4; ++r4) {
+          if (m4[r4] <= tau[g]) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const float v = acc[g][4 * r4 + e];
+              if (v <= tau[g]) {
+                const int off = base + n * 4;
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), cdrs, off, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b32(yb + r4 * 8 + e, cirs, off, 0, 0);
+                ++n;
+              }
+            }
           }
         }
         cnt[g] = n;
@@ -434,24 +455,27 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   // Step t (tile t in LDS buffer t%2, tile t+1 in flight in registers):
   //   issue the loads of tile t+2; MFMAs of tile t; write tile t+1 to LDS;
   //   select tile t-1 (its VALU work overlaps the MFMAs); barrier.
-  // Writing tile t+1 waits for every older vector-memory operation, which
-  // are then a full step old: the loads of tile t+1 and the candidate stores
-  // of the previous step's selection.
+  // The loads are issued unconditionally (tiles past the split read as zero
+  // through the buffer range check), so every step issues the same number of
+  // them and writing tile t+1 waits only for tile t+1's own loads, never for
+  // the tile t+2 loads issued at the top of the same step.  Offsets stay
+  // below 2^31: a split holds at most 1 GiB of rows, plus the 95 rows of
+  // look-ahead.
   const int64_t ntiles = yend > ybeg ? (yend - ybeg + kBY - 1) / kBY : 0;
   Staged stA, stB;
   f32x16 accA[QG], accB[QG];
   load_tile(stA, 0);
   store_tile(stA, 0);
-  if (ntiles > 1) load_tile(stB, 1);
+  load_tile(stB, 1);
   __syncthreads();
   // One step; `cur` is the LDS buffer of tile t, `ld` the free register set,
   // `st` the set holding tile t+1.
   auto one_step = [&](int64_t t, f32x16(&accC)[QG], f32x16(&accS)[QG], Staged& ld,
                       Staged& st) {
     const int cur = (int)(t & 1);
-    if (t + 2 < ntiles) load_tile(ld, t + 2);
+    load_tile(ld, t + 2);
     compute(accC, cur);
-    if (t + 1 < ntiles) store_tile(st, cur ^ 1);
+    store_tile(st, cur ^ 1);
     if (t > 0) select(accS, ybeg + (t - 1) * kBY, (int)(t - 1));
     __syncthreads();
   };
