@@ -55,10 +55,12 @@ typedef struct knn_stats {
   int64_t k_candidates;      /* k' kept per query before the exact re-rank */
   int64_t n_splits;          /* train splits searched in parallel */
   int64_t strategy_used;     /* KNN_STRATEGY_CHUNK_ON_TRAIN or _ON_TEST */
-  int64_t kernel_path;       /* 0 generic SIMT, 1 MFMA fp32, 2 MFMA bf16, 3 MFMA fp8 */
-  double prep_ms;            /* KNN_FLAG_TIMING: fills, norms, quantisation */
+  int64_t kernel_path;       /* 0 generic SIMT, 1 MFMA fp32, 2 MFMA bf16, 3 MFMA fp8, 4 low-d SIMT */
+  double prep_ms;            /* KNN_FLAG_TIMING: fills, norms, quantisation, sampling */
   double search_ms;          /* KNN_FLAG_TIMING: fused distance + candidate kernel */
-  double merge_ms;           /* KNN_FLAG_TIMING: merge + exact re-rank */
+  double merge_ms;           /* KNN_FLAG_TIMING: merge + exact re-rank (+ re-search) */
+  int64_t n_rescued;         /* queries searched a second time because the threshold
+                                guessed from a row sample was below their k'-th value */
 } knn_stats;
 
 /*

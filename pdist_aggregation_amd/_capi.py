@@ -22,7 +22,8 @@ STRATEGY_NAME = {v: k for k, v in STRATEGY.items()}
 PRECISION = {"exact": 0, "fp32": 0, "fp64": 0, "bf16": 1, "fp8": 2}
 FLAG_SQUARED = 1
 FLAG_TIMING = 2
-KERNEL_PATH = {0: "generic-simt", 1: "mfma-fp32", 2: "mfma-bf16", 3: "mfma-fp8"}
+KERNEL_PATH = {0: "generic-simt", 1: "mfma-fp32", 2: "mfma-bf16", 3: "mfma-fp8", 4: "lowd-simt"}
+ABI_VERSION = 2  # knn_abi_version() of the library this binding matches
 
 # Every symbol include/pdist_knn.h declares.
 EXPORTED = (
@@ -49,6 +50,7 @@ class KnnStats(ctypes.Structure):
         ("prep_ms", ctypes.c_double),
         ("search_ms", ctypes.c_double),
         ("merge_ms", ctypes.c_double),
+        ("n_rescued", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -103,6 +105,10 @@ def lib():
         h.knn_last_error.argtypes = []
         h.knn_abi_version.restype = ctypes.c_int
         h.knn_abi_version.argtypes = []
+        if h.knn_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {h.knn_abi_version()}, this binding "
+                               f"expects {ABI_VERSION}: rebuild with python -m "
+                               f"pdist_aggregation_amd.build")
         _LIB = h
     return _LIB
 

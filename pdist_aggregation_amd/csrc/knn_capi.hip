@@ -10,6 +10,7 @@
 // (src/_pdist_aggregation.pyx:236-448) or as a queue of SYCL kernels per
 // chunk pair (sycl_ext/parallel_knn.hpp:337-534).
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -70,10 +71,19 @@ struct Plan {
   int kc = 0, cap = 0, n_splits = 1, path = 0, dm = 0;
   int64_t rows_per_split = 0, n_qblocks = 0;
   size_t elem = 4;
+  // Sampled threshold (MFMA paths): a first launch over sample_regions row
+  // ranges of sample_rows rows, sample_stride apart, estimates each query's
+  // k'-th value as the sample_j-th smallest sampled minimum.
+  bool guess = false;
+  int guess_fail_stride = 0;  // > 0: force the guess of every stride-th query to fail
+  int sample_regions = 0, sample_j = 0;
+  int64_t sample_rows = 0, sample_stride = 0;
+  double sample_fraction = 0.0, guess_fail_bound = 0.0;
   // workspace offsets
   size_t off_scalars = 0, off_ynorm = 0, off_tau = 0, off_cand_d = 0, off_cand_i = 0,
          off_part_d = 0, off_part_i = 0, off_mu = 0, off_xq = 0, off_yq = 0, off_xnorm = 0,
-         off_xerr = 0, off_xscale = 0, off_yscale = 0, off_slots = 0, total = 0;
+         off_xerr = 0, off_xscale = 0, off_yscale = 0, off_slots = 0, off_sample = 0,
+         off_guess = 0, off_failed = 0, off_failed_qb = 0, total = 0;
 };
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
@@ -122,6 +132,61 @@ int choose_splits_slots(int64_t n_qblocks, int64_t n_train, int row_step, int kc
   for (int64_t s = 1; s <= smax; ++s)
     if (cost(s) <= 1.04 * best) return (int)s;
   return 1;
+}
+
+// P(Binomial(n, f) >= j).
+double binom_tail(int n, double f, int j) {
+  if (j <= 0) return 1.0;
+  if (j > n) return 0.0;
+  double pmf = std::pow(1.0 - f, n), cdf = 0.0;  // P(X = 0)
+  for (int i = 0; i < j; ++i) {
+    cdf += pmf;
+    pmf *= (double)(n - i) / (double)(i + 1) * f / (1.0 - f);
+  }
+  return std::max(0.0, 1.0 - cdf);
+}
+
+// Sampled-threshold plan.  The k'-1 rows ranked above a query's k'-th value
+// each fall in a sample of fraction f with probability f (rows in random
+// order), so the j-th smallest sampled value lies below the k'-th value --
+// the guess is too tight -- with probability P(Bin(k'-1, f) >= j) (minima
+// of 16 rows only raise the guess).  j is the smallest with that bound at
+// most kGuessFail; a query whose guess fails is searched again, so the bound
+// only governs speed.  PDIST_KNN_GUESS: "0" off, "1" on at any size,
+// "fail:S" on with the guess of every S-th query forced to fail (exercises
+// the re-search; "fail" = every query).
+constexpr int64_t kGuessMinRows = 131072;
+constexpr int kSampleShift = 5;  // sample 1/32 of the train rows
+constexpr double kGuessFail = 1e-7;
+
+void plan_guess(Plan& p, int64_t slots) {
+  const char* env = std::getenv("PDIST_KNN_GUESS");
+  const std::string mode = env ? env : "";
+  int fail_stride = 0;
+  if (mode == "fail") fail_stride = 1;
+  if (mode.rfind("fail:", 0) == 0) fail_stride = std::max(1, std::atoi(mode.c_str() + 5));
+  const bool forced = mode == "1" || fail_stride > 0;
+  if (mode == "0" || (!forced && p.n_train < kGuessMinRows)) return;
+  const int64_t tiles = knn::ceil_div(p.n_train, (int64_t)knn::kMfmaRowStep);
+  const int64_t want = std::max<int64_t>(1, knn::ceil_div(tiles, (int64_t)1 << kSampleShift));
+  int64_t regions = std::max<int64_t>(1, slots / std::max<int64_t>(p.n_qblocks, 1));
+  regions = std::min<int64_t>({regions, want, 64});
+  const int64_t per = knn::ceil_div(want, regions);                 // tiles per region
+  const int64_t stride = std::max<int64_t>(per, tiles / regions);   // tiles between regions
+  p.sample_regions = (int)regions;
+  p.sample_rows = per * knn::kMfmaRowStep;
+  p.sample_stride = stride * knn::kMfmaRowStep;
+  p.sample_fraction = std::min(1.0, (double)(regions * per) / (double)tiles);
+  int j = 1;
+  while (j <= knn::kSampleJ && binom_tail(p.kc - 1, p.sample_fraction, j) > kGuessFail) ++j;
+  if (j > knn::kSampleJ) {
+    if (!forced) return;
+    j = knn::kSampleJ;
+  }
+  p.sample_j = j;
+  p.guess_fail_bound = binom_tail(p.kc - 1, p.sample_fraction, j);
+  p.guess = true;
+  p.guess_fail_stride = fail_stride;
 }
 
 int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim, int64_t k,
@@ -197,6 +262,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   }
   p.n_qblocks = knn::ceil_div(std::max<int64_t>(n_test, 1), bq);
   int splits;
+  int64_t slots = 0;
   if (p.path == kPathGeneric) {
     splits = choose_splits(p.n_qblocks, n_train, step, p.kc, 2048, false);
   } else {
@@ -209,8 +275,8 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
                                        : p.path == kPathMfmaBF16 ? knn::kFmtBF16
                                                                  : knn::kFmtFP8,
                                        p.dm, p.cap);
-    splits = choose_splits_slots(p.n_qblocks, n_train, step, p.kc,
-                                 (int64_t)device_cus() * per_cu);
+    slots = (int64_t)device_cus() * per_cu;
+    splits = choose_splits_slots(p.n_qblocks, n_train, step, p.kc, slots);
     // The MFMA kernel addresses one split with 32-bit buffer offsets.
     if (p.path != kPathLowd) {
       const int64_t row_bytes = p.path == kPathMfmaF32 ? dim * 4
@@ -222,6 +288,8 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   }
   p.rows_per_split = knn::ceil_div(knn::ceil_div(n_train, splits), step) * step;
   p.n_splits = (int)knn::ceil_div(n_train, p.rows_per_split);
+  const bool mfma = p.path == kPathMfmaF32 || p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8;
+  if (mfma) plan_guess(p, slots);
 
   const size_t celem = (p.path == kPathGeneric || p.path == kPathLowd) ? elem : sizeof(float);
   const int64_t nq1 = std::max<int64_t>(n_test, 1);
@@ -240,8 +308,13 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   const size_t npart = (size_t)nq1 * p.n_splits * p.kc;
   p.off_part_d = take(npart * celem);
   p.off_part_i = take(npart * sizeof(int));
-  if (p.path == kPathMfmaF32 || p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8)
-    p.off_slots = take((size_t)nq1 * p.n_splits * sizeof(float) + 8);
+  if (mfma) p.off_slots = take((size_t)nq1 * p.n_splits * sizeof(float) + 8);
+  if (p.guess) {
+    p.off_sample = take((size_t)nq1 * p.sample_regions * 2 * knn::kSampleJ * sizeof(float));
+    p.off_guess = take((size_t)nq1 * sizeof(float));
+    p.off_failed = take((size_t)nq1 * sizeof(int));
+    p.off_failed_qb = take((size_t)p.n_qblocks * sizeof(int));
+  }
   if (p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8) {
     const size_t qe = p.path == kPathMfmaBF16 ? 2 : 1;
     p.off_mu = take(2 * (size_t)p.dm * sizeof(float));
@@ -337,6 +410,12 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
                                0ull, stream);
   knn::launch_fill_keys<float>(tau, n_test, knn::ordered_key(__builtin_huge_val()), stream);
   knn::MergeArgs<T> ma;
+  ma.failed = nullptr;
+  ma.failed_qblocks = nullptr;
+  ma.qblock_queries = 1;
+  ma.tau = tau;
+  ma.only_queries = nullptr;
+  knn::MfmaArgs fa;
   ma.xnorm = nullptr;
   ma.xerr = nullptr;
   ma.unit_roundoff = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
@@ -370,7 +449,6 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     ma.part_d = sa.part_d;
     ma.part_i = sa.part_i;
   } else if constexpr (sizeof(T) == 4) {
-    knn::MfmaArgs fa;
     fa.xscale = nullptr;
     fa.yscale = nullptr;
     fa.dm = p.dm;
@@ -381,6 +459,10 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     fa.n_qblocks = (int)p.n_qblocks;
     fa.n_splits = p.n_splits;
     fa.rows_per_split = p.rows_per_split;
+    fa.split_stride = p.rows_per_split;
+    fa.qblock_active = nullptr;
+    fa.query_active = nullptr;
+    fa.sample_out = nullptr;
     fa.cand_d = reinterpret_cast<float*>(ws + p.off_cand_d);
     fa.cand_i = reinterpret_cast<int*>(ws + p.off_cand_i);
     fa.tau = tau;
@@ -444,6 +526,29 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       ma.unit_roundoff = 0x1p-24;
       ma.acc_terms = 2 * p.dm + 4;
     }
+    // 2b. sampled threshold: minima over 1/32 of the rows -> guess -> tau
+    if (p.guess) {
+      auto* failed = reinterpret_cast<int*>(ws + p.off_failed);
+      auto* failed_qb = reinterpret_cast<int*>(ws + p.off_failed_qb);
+      knn::launch_fill_u32(reinterpret_cast<unsigned*>(failed), n_test, 0u, stream);
+      knn::launch_fill_u32(reinterpret_cast<unsigned*>(failed_qb), p.n_qblocks, 0u, stream);
+      knn::MfmaArgs sa = fa;
+      sa.n_splits = p.sample_regions;
+      sa.rows_per_split = p.sample_rows;
+      sa.split_stride = p.sample_stride;
+      sa.qslots = nullptr;
+      sa.sample_out = reinterpret_cast<float*>(ws + p.off_sample);
+      if (!knn::launch_mfma_sample(sa, stream)) {
+        if (owned) (void)hipFreeAsync(ws, stream);
+        return fail(KNN_EUNSUPPORTED, "no MFMA kernel for this feature count");
+      }
+      knn::launch_sample_guess(sa.sample_out, n_test, 2 * p.sample_regions, p.sample_j, tau,
+                               reinterpret_cast<float*>(ws + p.off_guess), p.guess_fail_stride,
+                               stream);
+      ma.failed = failed;
+      ma.failed_qblocks = failed_qb;
+      ma.qblock_queries = knn::mfma_block_queries(fa.fmt, p.dm);
+    }
     // 3. search
     timer.mark(1);
     if (!knn::launch_mfma_search(fa, stream)) {
@@ -468,6 +573,20 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.dist_out = dist_out;
   ma.idx_out = idx_out;
   knn::launch_merge_rerank<T>(ma, stream);
+  if constexpr (sizeof(T) == 4) {
+    // 5. queries whose sampled guess proved too tight: search again (only
+    // their query blocks, only them), from tau = +inf, and merge them.
+    if (ma.failed) {
+      knn::MfmaArgs ra = fa;
+      ra.qblock_active = ma.failed_qblocks;
+      ra.query_active = ma.failed;
+      knn::launch_mfma_search(ra, stream);
+      knn::MergeArgs<T> rm = ma;
+      rm.failed = nullptr;
+      rm.only_queries = ma.failed;
+      knn::launch_merge_rerank<T>(rm, stream);
+    }
+  }
   timer.mark(3);
 
   hipError_t e = hipGetLastError();
@@ -484,6 +603,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       return fail(KNN_EHIP, std::string("stats copy: ") + hipGetErrorString(e));
     }
     stats->n_uncertified = host_sc.n_uncertified;
+    stats->n_rescued = host_sc.n_rescued;
     if (timer.on) {
       stats->prep_ms = timer.elapsed(0, 1);
       stats->search_ms = timer.elapsed(1, 2);
@@ -602,6 +722,6 @@ size_t knn_argkmin_workspace_size(int dtype_bytes, int64_t n_train, int64_t n_te
 
 const char* knn_last_error(void) { return g_last_error.c_str(); }
 
-int knn_abi_version(void) { return 1; }
+int knn_abi_version(void) { return 2; }
 
 }  // extern "C"

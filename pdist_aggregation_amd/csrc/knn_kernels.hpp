@@ -23,7 +23,7 @@ struct DeviceScalars {
   unsigned long long max_ynorm_key;  // ordered key of max_j |y_j|^2 (as used by the search)
   unsigned long long max_yerr_key;   // ordered key of max_j |y_j - q(y_j)| (low precision)
   unsigned int n_uncertified;
-  unsigned int pad;
+  unsigned int n_rescued;            // queries searched again after a too-tight sampled threshold
 };
 
 // Everything the fused search kernels need.  T is the element type of the
@@ -83,12 +83,34 @@ struct MfmaArgs {
   // kSlotJ * (slot_rank+1) >= k'; nullptr disables it.
   float* qslots;
   int slot_rank;
+  // Split s covers rows [s * split_stride, s * split_stride + rows_per_split)
+  // (split_stride == rows_per_split for the search; larger for sampling).
+  int64_t split_stride;
+  // Re-search of selected queries: workgroups of query blocks whose
+  // qblock_active entry is 0 exit at once, queries whose query_active entry
+  // is 0 neither collect candidates nor write lists.  nullptr = all active.
+  const int* qblock_active;
+  const int* query_active;
+  // Sampling launch: [n_q][n_splits][2][kSampleJ] ascending smallest
+  // per-tile minima seen by each lane.
+  float* sample_out;
 };
 
 constexpr int kSlotJ = 4;
+constexpr int kSampleJ = 16;  // per-lane minima kept by the sampling launch
 
 // Returns false when no instantiation matches (fmt, dm).
 bool launch_mfma_search(const MfmaArgs& a, hipStream_t s);
+// The sampling launch of the same kernel: rows [s * split_stride, + rows_per_split)
+// of each split s, writing sample_out; no candidate lists.
+bool launch_mfma_sample(const MfmaArgs& a, hipStream_t s);
+// guess[q] = the j-th smallest of the sampled minima of query q (ascending
+// runs of kSampleJ, n_runs per query); tau[q] <- min(tau[q], key(guess[q])).
+// fail_stride > 0 forces guess -inf on queries q % fail_stride == 0, which
+// then fail the check and are searched again (exercises that path).
+void launch_sample_guess(const float* sample_out, int64_t n_q, int n_runs, int j,
+                         unsigned long long* tau, float* guess, int fail_stride, hipStream_t s);
+void launch_fill_u32(unsigned int* p, int64_t n, unsigned int v, hipStream_t s);
 // Queries per workgroup of the (fmt, dm) instantiation (4 waves x 32 x groups).
 int mfma_block_queries(int fmt, int dm);
 // Resident workgroups per CU of that instantiation (2 if it cannot be queried).
@@ -155,6 +177,17 @@ struct MergeArgs {
   DeviceScalars* sc;
   T* dist_out;        // n_q x k
   int64_t* idx_out;   // n_q x k
+  // Sampled-threshold check (nullptr: off).  The search started from a
+  // guessed threshold; a query with fewer than min(k', n_t) merged candidates
+  // may have lost rows to it: it gets failed[q] = 1, its query block
+  // (q / qblock_queries) failed_qblocks = 1 and tau[q] = key(+inf), and its
+  // outputs are left to the re-search.
+  int* failed;
+  int* failed_qblocks;
+  int qblock_queries;
+  unsigned long long* tau;
+  // Re-search merge: only queries with only_queries[q] != 0 (nullptr: all).
+  const int* only_queries;
 };
 
 template <class T>

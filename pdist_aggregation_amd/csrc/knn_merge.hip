@@ -29,6 +29,45 @@ __global__ void fill_keys_kernel(unsigned long long* p, int64_t n, unsigned long
     p[i] = v;
 }
 
+__global__ void fill_u32_kernel(unsigned int* p, int64_t n, unsigned int v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// One thread per query: the j-th smallest of the query's sampled minima
+// (n_runs ascending runs of kSampleJ values), used as a guessed threshold.
+__global__ __launch_bounds__(kThreads) void sample_guess_kernel(const float* __restrict__ so,
+                                                                int64_t n_q, int n_runs, int j,
+                                                                unsigned long long* tau,
+                                                                float* guess, int fail_stride) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_q) return;
+  float b[kSampleJ];
+#pragma unroll
+  for (int i = 0; i < kSampleJ; ++i) b[i] = __builtin_huge_valf();
+  const float* p = so + (size_t)q * n_runs * kSampleJ;
+  for (int r = 0; r < n_runs; ++r) {
+    for (int i = 0; i < kSampleJ; ++i) {
+      const float m = p[r * kSampleJ + i];
+      if (!(m < b[kSampleJ - 1])) break;  // the rest of an ascending run is no better
+#pragma unroll
+      for (int t = kSampleJ - 1; t > 0; --t) b[t] = fminf(b[t], fmaxf(b[t - 1], m));
+      b[0] = fminf(b[0], m);
+    }
+  }
+  float g = __builtin_huge_valf();
+#pragma unroll
+  for (int i = 0; i < kSampleJ; ++i)
+    if (i == j - 1) g = b[i];
+  if (fail_stride > 0 && q % fail_stride == 0) g = -__builtin_huge_valf();
+  guess[q] = g;
+  if (g < __builtin_huge_valf()) {
+    const unsigned long long key = ordered_key((double)g);
+    if (key < tau[q]) tau[q] = key;
+  }
+}
+
 // One wave per row: |y_j|^2 accumulated in T (the precision the search uses)
 // and the maximum over all rows as an ordered key.
 template <class T>
@@ -75,6 +114,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   __shared__ double xnorm_s;
 
   const int64_t q = blockIdx.x;
+  if (a.only_queries && a.only_queries[q] == 0) return;  // re-search merge: not selected
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const T* pd = a.part_d + (size_t)q * total;
   const int* pi = a.part_i + (size_t)q * total;
@@ -84,6 +124,22 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     si[e] = ok ? pi[e] : INT_MAX;
   }
   block_bitonic_sort(sd, si, P);  // ends with a barrier
+
+  // Sampled-threshold check: with at least min(k', n_t) candidates every row
+  // the guess dropped has a value above the k'-th kept one, so nothing is
+  // lost; with fewer, the query is searched again from an infinite threshold.
+  if (a.failed) {
+    const int need = (int64_t)a.kc < a.n_t ? a.kc : (int)a.n_t;
+    if (si[need - 1] == INT_MAX) {  // same LDS value for every thread
+      if (tid == 0) {
+        a.failed[q] = 1;
+        atomicOr(&a.failed_qblocks[q / a.qblock_queries], 1);
+        a.tau[q] = ordered_key(__builtin_huge_val());
+        atomicAdd(&a.sc->n_rescued, 1u);
+      }
+      return;
+    }
+  }
 
   // Exact FP64 direct-difference distances of the best k' candidates,
   // one wave per candidate, lanes across features.
@@ -160,6 +216,19 @@ void launch_row_norms(const T* Y, int64_t n, int d, T* out, DeviceScalars* sc, h
   const int64_t blocks = ceil_div(n, 4) < 8192 ? ceil_div(n, 4) : 8192;
   hipLaunchKernelGGL(row_norms_kernel<T>, dim3((unsigned)blocks), dim3(kThreads), 0, s, Y, n, d,
                      out, sc);
+}
+
+void launch_fill_u32(unsigned int* p, int64_t n, unsigned int v, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = ceil_div(n, kThreads) < 4096 ? ceil_div(n, kThreads) : 4096;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, p, n, v);
+}
+
+void launch_sample_guess(const float* sample_out, int64_t n_q, int n_runs, int j,
+                         unsigned long long* tau, float* guess, int fail_stride, hipStream_t s) {
+  if (n_q <= 0) return;
+  hipLaunchKernelGGL(sample_guess_kernel, dim3((unsigned)ceil_div(n_q, kThreads)), dim3(kThreads),
+                     0, s, sample_out, n_q, n_runs, j, tau, guess, fail_stride);
 }
 
 size_t merge_lds_bytes(int n_splits, int kc, size_t elem) {

@@ -71,7 +71,13 @@ constexpr int row_pitch() {
   return DM * Fmt<FMT>::esize + 16;
 }
 
-template <int FMT, int DM, int QG>
+// MODE kModeSearch: the candidate search.  MODE kModeSample: the same
+// sweep over sampled row ranges, keeping only each lane's kSampleJ smallest
+// per-tile minima in registers (a threshold estimate, no lists).
+constexpr int kModeSearch = 0;
+constexpr int kModeSample = 1;
+
+template <int FMT, int DM, int QG, int MODE>
 __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   using F = Fmt<FMT>;
   constexpr int PITCH = row_pitch<FMT, DM>();
@@ -103,8 +109,9 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     split = w_item / a.n_qblocks;
     qb = w_item % a.n_qblocks;
   }
+  if (a.qblock_active && a.qblock_active[qb] == 0) return;  // re-search: block not selected
   const int64_t q0 = (int64_t)qb * BQ;
-  const int64_t ybeg = (int64_t)split * a.rows_per_split;
+  const int64_t ybeg = (int64_t)split * a.split_stride;
   const int64_t yend = ybeg + a.rows_per_split < a.n_t ? ybeg + a.rows_per_split : a.n_t;
   const size_t blk = (size_t)split * a.n_qblocks + qb;
   float* cand_d = a.cand_d + blk * BQ * a.cap;
@@ -117,11 +124,13 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   typename F::breg xr[QG][NSTEP];
   int xsc[QG];  // FP8: E8M0 scale of -2 x (the row scale times 2)
   float tau[QG];
+  bool qact[QG];  // query exists and takes part in this launch
 #pragma unroll
   for (int g = 0; g < QG; ++g) {
     const int64_t q = q0 + wq0 + g * 32 + c;
     const bool qok = q < a.n_q;
-    tau[g] = qok ? FLT_MAX : -__builtin_huge_valf();
+    qact[g] = qok && (!a.query_active || a.query_active[q] != 0);
+    tau[g] = qact[g] ? FLT_MAX : -__builtin_huge_valf();
     if constexpr (FMT == kFmtF32) {
       const float* xrow = reinterpret_cast<const float*>(a.X) + q * a.ldx;
 #pragma unroll
@@ -174,7 +183,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   for (int g = 0; g < QG; ++g) {
     cnt[g] = 0;
     const int64_t q = q0 + wq0 + g * 32 + c;
-    if (q < a.n_q)  // start from the best threshold known (seeded or shared)
+    if (MODE == kModeSearch && qact[g])  // start from the best threshold known
       tau[g] = fminf(tau[g],
                      (float)from_ordered_key(load_agent(&a.tau[q])));
   }
@@ -427,7 +436,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
 #pragma unroll
       for (int g = 0; g < QG; ++g) {
         const int64_t q = q0 + wq0 + g * 32 + c;
-        if (q < a.n_q) {
+        if (qact[g]) {
           float t = (float)from_ordered_key(load_agent(&a.tau[q]));
           if (a.qslots) {
             float r[kSlotJ];
@@ -462,33 +471,74 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   // below 2^31: a split holds at most 1 GiB of rows, plus the 95 rows of
   // look-ahead.
   const int64_t ntiles = yend > ybeg ? (yend - ybeg + kBY - 1) / kBY : 0;
-  Staged stA, stB;
-  f32x16 accA[QG], accB[QG];
-  load_tile(stA, 0);
-  store_tile(stA, 0);
-  load_tile(stB, 1);
-  __syncthreads();
-  // One step; `cur` is the LDS buffer of tile t, `ld` the free register set,
-  // `st` the set holding tile t+1.
-  auto one_step = [&](int64_t t, f32x16(&accC)[QG], f32x16(&accS)[QG], Staged& ld,
-                      Staged& st) {
-    const int cur = (int)(t & 1);
-    load_tile(ld, t + 2);
-    compute(accC, cur);
-    store_tile(st, cur ^ 1);
-    if (t > 0) select(accS, ybeg + (t - 1) * kBY, (int)(t - 1));
+  // `sel(acc, y0, step)` consumes one finished tile.
+  auto sweep = [&](auto&& sel) {
+    Staged stA, stB;
+    f32x16 accA[QG], accB[QG];
+    load_tile(stA, 0);
+    store_tile(stA, 0);
+    load_tile(stB, 1);
     __syncthreads();
+    // One step; `cur` is the LDS buffer of tile t, `ld` the free register
+    // set, `st` the set holding tile t+1.
+    auto one_step = [&](int64_t t, f32x16(&accC)[QG], f32x16(&accS)[QG], Staged& ld,
+                        Staged& st) {
+      const int cur = (int)(t & 1);
+      load_tile(ld, t + 2);
+      compute(accC, cur);
+      store_tile(st, cur ^ 1);
+      if (t > 0) sel(accS, ybeg + (t - 1) * kBY, (int)(t - 1));
+      __syncthreads();
+    };
+    for (int64_t t = 0; t < ntiles; t += 2) {
+      one_step(t, accA, accB, stA, stB);
+      if (t + 1 < ntiles) one_step(t + 1, accB, accA, stB, stA);
+    }
+    if (ntiles > 0) {
+      if ((ntiles - 1) & 1)
+        sel(accB, ybeg + (ntiles - 1) * kBY, (int)(ntiles - 1));
+      else
+        sel(accA, ybeg + (ntiles - 1) * kBY, (int)(ntiles - 1));
+    }
   };
-  for (int64_t t = 0; t < ntiles; t += 2) {
-    one_step(t, accA, accB, stA, stB);
-    if (t + 1 < ntiles) one_step(t + 1, accB, accA, stB, stA);
+
+  if constexpr (MODE == kModeSample) {
+    // Each lane keeps the kSampleJ smallest of its per-tile minima (16 rows
+    // of one query per tile), sorted ascending in registers.  Inserting m
+    // into the sorted run: b[i] <- min(b[i], max(b[i-1], m)), b[0] <- min(b[0], m).
+    float best[QG][kSampleJ];
+#pragma unroll
+    for (int g = 0; g < QG; ++g)
+#pragma unroll
+      for (int i = 0; i < kSampleJ; ++i) best[g][i] = __builtin_huge_valf();
+    sweep([&](f32x16(&acc)[QG], int64_t, int) {
+#pragma unroll
+      for (int g = 0; g < QG; ++g) {
+        float m = acc[g][0];
+#pragma unroll
+        for (int i = 1; i < 16; ++i) m = fminf(m, acc[g][i]);
+        if (__any(qact[g] && m < best[g][kSampleJ - 1])) {
+#pragma unroll
+          for (int i = kSampleJ - 1; i > 0; --i)
+            best[g][i] = fminf(best[g][i], fmaxf(best[g][i - 1], m));
+          best[g][0] = fminf(best[g][0], m);
+        }
+      }
+    });
+#pragma unroll
+    for (int g = 0; g < QG; ++g) {
+      const int64_t q = q0 + wq0 + g * 32 + c;
+      if (!qact[g]) continue;
+      float* o = a.sample_out + (((size_t)q * a.n_splits + split) * 2 + h) * kSampleJ;
+#pragma unroll
+      for (int i = 0; i < kSampleJ; i += 4)
+        *reinterpret_cast<f32x4*>(o + i) =
+            f32x4{best[g][i], best[g][i + 1], best[g][i + 2], best[g][i + 3]};
+    }
+    return;
   }
-  if (ntiles > 0) {
-    if ((ntiles - 1) & 1)
-      select(accB, ybeg + (ntiles - 1) * kBY, (int)(ntiles - 1));
-    else
-      select(accA, ybeg + (ntiles - 1) * kBY, (int)(ntiles - 1));
-  }
+
+  sweep(select);
 
   // ---- final: sort every list of this wave, write the split's slot --------
 #pragma unroll
@@ -499,6 +549,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     const int ql = wq0 + qq;
     const int64_t q = q0 + ql;
     if (q >= a.n_q) break;
+    if (a.query_active && a.query_active[q] == 0) continue;  // uniform across the wave
     const int n0 = cnt_s[2 * ql], n1 = cnt_s[2 * ql + 1];
     const int n = n0 + n1;
     if (n > 0) sort_query(ql, n0, n1);
@@ -512,18 +563,20 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   }
 }
 
-template <int FMT, int DM, int QG>
+template <int FMT, int DM, int QG, int MODE>
 void launch_one(const MfmaArgs& a, hipStream_t s) {
-  const size_t lds = (size_t)4 * a.cap * 8;
+  const size_t lds = MODE == kModeSearch ? (size_t)4 * a.cap * 8 : 0;
   const unsigned blocks = (unsigned)((a.n_qblocks * a.n_splits + 7) / 8 * 8);
-  hipLaunchKernelGGL((mfma_search_kernel<FMT, DM, QG>), dim3(blocks), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL((mfma_search_kernel<FMT, DM, QG, MODE>), dim3(blocks), dim3(kThreads), lds,
+                     s, a);
 }
 
 template <int FMT, int DM, int QG>
 int occupancy_one(int cap) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mfma_search_kernel<FMT, DM, QG>, kThreads,
-                                                   (size_t)4 * cap * 8) != hipSuccess || n <= 0)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mfma_search_kernel<FMT, DM, QG, kModeSearch>,
+                                                   kThreads, (size_t)4 * cap * 8) != hipSuccess ||
+      n <= 0)
     return 2;
   return n;
 }
@@ -557,11 +610,12 @@ int mfma_blocks_per_cu(int fmt, int dm, int cap) {
   return 2;
 }
 
-bool launch_mfma_search(const MfmaArgs& a, hipStream_t s) {
-#define KNN_MFMA_CASE(F, D)                         \
-  if (a.fmt == F && a.dm == D) {                    \
-    launch_one<F, D, groups_for(F, D)>(a, s);       \
-    return true;                                    \
+template <int MODE>
+static bool launch_mode(const MfmaArgs& a, hipStream_t s) {
+#define KNN_MFMA_CASE(F, D)                              \
+  if (a.fmt == F && a.dm == D) {                         \
+    launch_one<F, D, groups_for(F, D), MODE>(a, s);      \
+    return true;                                         \
   }
   KNN_MFMA_CASE(kFmtF32, 32)
   KNN_MFMA_CASE(kFmtF32, 64)
@@ -576,5 +630,9 @@ bool launch_mfma_search(const MfmaArgs& a, hipStream_t s) {
 #undef KNN_MFMA_CASE
   return false;
 }
+
+bool launch_mfma_search(const MfmaArgs& a, hipStream_t s) { return launch_mode<kModeSearch>(a, s); }
+
+bool launch_mfma_sample(const MfmaArgs& a, hipStream_t s) { return launch_mode<kModeSample>(a, s); }
 
 }  // namespace knn

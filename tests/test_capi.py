@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     lib = _capi.lib()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.knn_abi_version() == 1
+    assert lib.knn_abi_version() == _capi.ABI_VERSION
 
 
 def test_workspace_size():

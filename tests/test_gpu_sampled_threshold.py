@@ -1,0 +1,122 @@
+"""Sampled-threshold start and re-search of the MFMA paths (needs a GPU).
+
+The MFMA search first sweeps 1/32 of the train rows, keeps per-lane minima
+and starts every query from a guessed threshold (the j-th smallest sampled
+minimum).  A query left with fewer than k' candidates is searched again from
+an infinite threshold.  PDIST_KNN_GUESS forces the sampling on at small sizes
+("1") and forces chosen guesses to fail ("fail", "fail:S" = every S-th
+query), so both the normal and the re-search path are compared with the FP64
+oracle here.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import pdist_aggregation_amd as pa
+from knn_checks import EPS_F32, assert_knn_parity, assert_sorted_unique
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _run(Y, X, k, precision, **kw):
+    (dist, idx), _, st = pa.parallel_knn(Y, X, k, return_distance=True, squared=True,
+                                         precision=precision, return_stats=True, **kw)
+    return dist, idx, st
+
+
+def _check(X, Y, dist, idx, k, precision, st):
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    if precision == "exact":
+        assert_knn_parity(X, Y, dist, idx, k, EPS_F32)
+        return
+    ref = oracle.exact_knn_f64(X, Y, k, return_boundary=True)
+    rec = oracle.recall_at_k(idx, ref[1])
+    _say(f"    {precision}: recall@{k}={rec:.5f} uncertified={st['n_uncertified']}")
+    assert rec >= (0.999 if precision == "bf16" else 0.99)
+    got = oracle._direct_sqdist(X, Y, idx)
+    np.testing.assert_allclose(dist, got, rtol=1e-6)
+    if st["n_uncertified"] == 0:
+        assert_knn_parity(X, Y, dist, idx, k, EPS_F32, ref=ref)
+
+
+CASES = [("exact", 64, 10), ("exact", 128, 10), ("exact", 32, 1), ("bf16", 128, 10),
+         ("bf16", 50, 32), ("fp8", 128, 10)]
+
+
+@pytest.mark.parametrize("precision,d,k", CASES)
+def test_sampled_guess_matches_oracle(require_gpu, monkeypatch, precision, d, k):
+    monkeypatch.setenv("PDIST_KNN_GUESS", "1")
+    rs = np.random.RandomState(d * 7 + k)
+    Y = rs.rand(40000, d).astype(np.float32)
+    X = rs.rand(700, d).astype(np.float32)
+    _say(f"[guess] {precision} d={d} k={k} 40000 x 700")
+    dist, idx, st = _run(Y, X, k, precision)
+    _say(f"    stats {st}")
+    assert st["kernel_path"] in (1, 2, 3)
+    _check(X, Y, dist, idx, k, precision, st)
+
+
+@pytest.mark.parametrize("precision,d,k", CASES)
+def test_every_guess_failing_is_searched_again(require_gpu, monkeypatch, precision, d, k):
+    monkeypatch.setenv("PDIST_KNN_GUESS", "fail")
+    rs = np.random.RandomState(d * 11 + k)
+    Y = rs.rand(30000, d).astype(np.float32)
+    X = rs.rand(513, d).astype(np.float32)
+    _say(f"[guess fail] {precision} d={d} k={k} 30000 x 513")
+    dist, idx, st = _run(Y, X, k, precision)
+    assert st["n_rescued"] == X.shape[0]
+    _check(X, Y, dist, idx, k, precision, st)
+
+
+@pytest.mark.parametrize("stride", [500, 3, 64])
+def test_some_guesses_failing(require_gpu, monkeypatch, stride):
+    """Only every stride-th query fails: the re-search must skip whole query
+    blocks without one (stride 500 leaves the last block out) and leave the
+    other queries of a selected block untouched."""
+    monkeypatch.setenv("PDIST_KNN_GUESS", f"fail:{stride}")
+    rs = np.random.RandomState(stride)
+    Y = rs.rand(25000, 64).astype(np.float32)
+    X = rs.rand(700, 64).astype(np.float32)
+    _say(f"[guess fail:{stride}] exact d=64 25000 x 700")
+    dist, idx, st = _run(Y, X, 10, "exact")
+    assert st["n_rescued"] == len(range(0, 700, stride))
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, 10, EPS_F32)
+
+
+def test_duplicated_rows_with_guess(require_gpu, monkeypatch):
+    """Every distinct train row repeated 40 times along the set: a query's k'
+    nearest are exact ties, sampled copies put the guess on the tie value."""
+    monkeypatch.setenv("PDIST_KNN_GUESS", "1")
+    rs = np.random.RandomState(5)
+    base = rs.rand(1000, 64).astype(np.float32)
+    Y = np.tile(base, (40, 1))
+    X = (base[rs.randint(0, 1000, 300)] + 1e-3 * rs.randn(300, 64)).astype(np.float32)
+    dist, idx, st = _run(Y, X, 10, "exact")
+    _say(f"    duplicates: stats {st}")
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, 10, EPS_F32)
+
+
+def test_guess_default_off_below_size_and_repeatable(require_gpu, monkeypatch):
+    """Without the override small searches take no sample; two calls on the
+    same inputs give identical results (re-search or not)."""
+    monkeypatch.delenv("PDIST_KNN_GUESS", raising=False)
+    rs = np.random.RandomState(9)
+    Y = rs.rand(20000, 64).astype(np.float32)
+    X = rs.rand(300, 64).astype(np.float32)
+    d0, i0, st0 = _run(Y, X, 10, "exact")
+    monkeypatch.setenv("PDIST_KNN_GUESS", "1")
+    d1, i1, st1 = _run(Y, X, 10, "exact")
+    d2, i2, _ = _run(Y, X, 10, "exact")
+    assert st0["n_rescued"] == 0
+    np.testing.assert_array_equal(i1, i2)
+    np.testing.assert_array_equal(d1, d2)
+    np.testing.assert_array_equal(i0, i1)
