@@ -308,7 +308,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   const size_t npart = (size_t)nq1 * p.n_splits * p.kc;
   p.off_part_d = take(npart * celem);
   p.off_part_i = take(npart * sizeof(int));
-  if (mfma) p.off_slots = take((size_t)nq1 * p.n_splits * sizeof(float) + 8);
+  if (mfma && !p.guess) p.off_slots = take((size_t)nq1 * knn::align8(p.n_splits) * sizeof(float));
   if (p.guess) {
     p.off_sample = take((size_t)nq1 * p.sample_regions * 2 * knn::kSampleJ * sizeof(float));
     p.off_guess = take((size_t)nq1 * sizeof(float));
@@ -470,15 +470,16 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     fa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
     fa.ynorm = reinterpret_cast<float*>(ynorm);
     // Shared-threshold slots (+inf = nothing published), used when at least
-    // kSlotJ splits exist.
+    // kSlotJ splits exist and no sampled threshold is in use: with one, lists
+    // rarely reach k' entries, so slots are seldom published, and scanning
+    // them (n_splits loads per query every 32 steps) costs more than it saves.
     fa.qslots = nullptr;
     fa.slot_rank = (int)knn::ceil_div(p.kc, knn::kSlotJ) - 1;
-    if (p.n_splits >= knn::kSlotJ) {
+    fa.slot_stride = knn::align8(p.n_splits);
+    if (p.n_splits >= knn::kSlotJ && !p.guess) {
       fa.qslots = reinterpret_cast<float*>(ws + p.off_slots);
-      const unsigned long long two_inf = 0x7f8000007f800000ull;
-      knn::launch_fill_keys<float>(reinterpret_cast<unsigned long long*>(fa.qslots),
-                                   knn::ceil_div((int64_t)n_test * p.n_splits, 2), two_inf,
-                                   stream);
+      knn::launch_fill_u32(reinterpret_cast<unsigned*>(fa.qslots), n_test * fa.slot_stride,
+                           0x7f800000u, stream);  // +inf
     }
     if (p.path == kPathMfmaF32) {
       // 2. norms
@@ -580,7 +581,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       knn::MfmaArgs ra = fa;
       ra.qblock_active = ma.failed_qblocks;
       ra.query_active = ma.failed;
-      knn::launch_mfma_search(ra, stream);
+      knn::launch_mfma_research(ra, stream);
       knn::MergeArgs<T> rm = ma;
       rm.failed = nullptr;
       rm.only_queries = ma.failed;

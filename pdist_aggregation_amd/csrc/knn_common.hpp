@@ -155,5 +155,6 @@ __host__ __device__ inline int next_pow2(int v) {
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
+__host__ __device__ inline int align8(int v) { return (v + 7) & ~7; }
 
 }  // namespace knn

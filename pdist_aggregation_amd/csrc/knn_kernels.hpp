@@ -78,17 +78,19 @@ struct MfmaArgs {
   unsigned long long* tau;
   float* part_d;
   int* part_i;
-  // Cross-split threshold sharing: qslots[q][split] (init +inf) receives the
-  // split's (slot_rank+1)-th smallest value at each list compaction, with
+  // Cross-split threshold sharing: qslots[q * slot_stride + split] (init
+  // +inf; slot_stride = n_splits rounded up to 8) receives the split's
+  // (slot_rank+1)-th smallest value at each list compaction, with
   // kSlotJ * (slot_rank+1) >= k'; nullptr disables it.
   float* qslots;
   int slot_rank;
+  int slot_stride;
   // Split s covers rows [s * split_stride, s * split_stride + rows_per_split)
   // (split_stride == rows_per_split for the search; larger for sampling).
   int64_t split_stride;
-  // Re-search of selected queries: workgroups of query blocks whose
-  // qblock_active entry is 0 exit at once, queries whose query_active entry
-  // is 0 neither collect candidates nor write lists.  nullptr = all active.
+  // Re-search launch only: workgroups of query blocks whose qblock_active
+  // entry is 0 exit at once, queries whose query_active entry is 0 neither
+  // collect candidates nor write lists.
   const int* qblock_active;
   const int* query_active;
   // Sampling launch: [n_q][n_splits][2][kSampleJ] ascending smallest
@@ -101,6 +103,8 @@ constexpr int kSampleJ = 16;  // per-lane minima kept by the sampling launch
 
 // Returns false when no instantiation matches (fmt, dm).
 bool launch_mfma_search(const MfmaArgs& a, hipStream_t s);
+// The re-search launch (qblock_active and query_active must be set).
+bool launch_mfma_research(const MfmaArgs& a, hipStream_t s);
 // The sampling launch of the same kernel: rows [s * split_stride, + rows_per_split)
 // of each split s, writing sample_out; no candidate lists.
 bool launch_mfma_sample(const MfmaArgs& a, hipStream_t s);

@@ -97,6 +97,11 @@ __global__ __launch_bounds__(kThreads) void row_norms_kernel(const T* __restrict
   }
 }
 
+// Candidates left after the threshold filter up to which the k' best are
+// found by ranking (each thread counts the entries ahead of its own, no
+// barriers); more than that are bitonic-sorted.
+constexpr int kRankMax = 128;
+
 // One workgroup per query.
 template <class T>
 __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) {
@@ -111,26 +116,69 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   int* si = reinterpret_cast<int*>(dyn_smem + o_si);
   double* ed = reinterpret_cast<double*>(dyn_smem + o_ed);
   int* ei = reinterpret_cast<int*>(dyn_smem + o_ei);
-  __shared__ double xnorm_s;
+  __shared__ double xnorm_s, akc_s;
+  __shared__ int n_s, full_s;
 
   const int64_t q = blockIdx.x;
   if (a.only_queries && a.only_queries[q] == 0) return;  // re-search merge: not selected
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const T* pd = a.part_d + (size_t)q * total;
   const int* pi = a.part_i + (size_t)q * total;
-  for (int e = tid; e < P; e += kThreads) {
-    const bool ok = e < total && pi[e] >= 0 && pi[e] < a.n_t;  // never index past Y
-    sd[e] = ok ? pd[e] : pos_inf<T>();
-    si[e] = ok ? pi[e] : INT_MAX;
+  // The query's final shared threshold bounds its k'-th value whenever the
+  // search kept k' entries at or below it (a split reached k' entries there,
+  // or every entry passed the sampled guess): entries above it are dropped
+  // here.  Without k' entries at or below it the query fails the check below.
+  const T tq = a.tau ? (T)from_ordered_key(a.tau[q]) : pos_inf<T>();
+  if (tid == 0) n_s = 0;
+  __syncthreads();
+  for (int e = tid; e < total; e += kThreads) {
+    const int id = pi[e];
+    const T v = pd[e];
+    if (id >= 0 && id < a.n_t && v <= tq) {  // never index past Y
+      const int at = atomicAdd(&n_s, 1);
+      sd[at] = v;
+      si[at] = id;
+    }
   }
-  block_bitonic_sort(sd, si, P);  // ends with a barrier
+  __syncthreads();
+  const int n = n_s;
+  // The k' best, ascending: ranked into the (still unused) exact-distance
+  // arrays, or the head of the sorted staging arrays.
+  T* od = n <= kRankMax ? reinterpret_cast<T*>(ed) : sd;
+  int* oi = n <= kRankMax ? ei : si;
+  if (n <= kRankMax) {
+    // (value, index) keys are distinct (a train row is in one split only), so
+    // ranks are a permutation.
+    for (int e = tid; e < n; e += kThreads) {
+      const T v = sd[e];
+      const int id = si[e];
+      int rank = 0;
+      for (int j = 0; j < n; ++j) rank += key_less(sd[j], si[j], v, id) ? 1 : 0;
+      if (rank < a.kc) {
+        od[rank] = v;
+        oi[rank] = id;
+      }
+    }
+  } else {
+    const int Pn = next_pow2(n);
+    for (int e = n + tid; e < Pn; e += kThreads) {
+      sd[e] = pos_inf<T>();
+      si[e] = INT_MAX;
+    }
+    block_bitonic_sort(sd, si, Pn);
+  }
+  for (int j = n + tid; j < a.kc; j += kThreads) {
+    od[j] = pos_inf<T>();
+    oi[j] = INT_MAX;
+  }
+  __syncthreads();
 
   // Sampled-threshold check: with at least min(k', n_t) candidates every row
   // the guess dropped has a value above the k'-th kept one, so nothing is
   // lost; with fewer, the query is searched again from an infinite threshold.
   if (a.failed) {
     const int need = (int64_t)a.kc < a.n_t ? a.kc : (int)a.n_t;
-    if (si[need - 1] == INT_MAX) {  // same LDS value for every thread
+    if (n < need) {  // n_s: the same value for every thread
       if (tid == 0) {
         a.failed[q] = 1;
         atomicOr(&a.failed_qblocks[q / a.qblock_queries], 1);
@@ -140,22 +188,33 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
       return;
     }
   }
+  if (tid == 0) {
+    full_s = oi[a.kc - 1] != INT_MAX;
+    akc_s = full_s ? (double)od[a.kc - 1] : 0.0;
+  }
+  __syncthreads();  // od is overwritten below (it may share storage with ed)
 
-  // Exact FP64 direct-difference distances of the best k' candidates,
-  // one wave per candidate, lanes across features.
+  // Exact FP64 direct-difference distances of the best k' candidates: tpc
+  // consecutive threads per candidate, each over every tpc-th feature (so
+  // every candidate row's loads are in flight together), then a shuffle
+  // reduction over the tpc lanes.
   const T* xr = a.X + q * a.d;
-  for (int j = w; j < KP; j += kThreads / 64) {
-    const int id = j < a.kc ? si[j] : INT_MAX;
+  const int tpc = KP >= kThreads ? 1 : (kThreads / KP < 64 ? kThreads / KP : 64);  // 2^i <= 64
+  const int per_pass = kThreads / tpc;
+  for (int j0 = 0; j0 < KP; j0 += per_pass) {
+    const int j = j0 + tid / tpc, c = tid % tpc;
+    const int id = j < a.kc ? oi[j] : INT_MAX;
     double s = 0.0;
     if (id != INT_MAX) {
       const T* yr = a.Y + (int64_t)id * a.d;
-      for (int p = lane; p < a.d; p += 64) {
+      for (int p = c; p < a.d; p += tpc) {
         const double df = (double)xr[p] - (double)yr[p];
         s = fma(df, df, s);
       }
-      s = wave_sum(s);
     }
-    if (lane == 0) {
+    for (int o = tpc >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __syncthreads();  // every oi[] read of this pass precedes the ed/ei writes (shared storage)
+    if (c == 0 && j < KP) {
       ed[j] = id != INT_MAX ? s : __builtin_huge_val();
       ei[j] = id;
     }
@@ -184,8 +243,8 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   // exceeds the k-th exact distance the k returned rows are the exact answer.
   if (tid == 0) {
     bool certified = a.kc >= a.n_t;
-    if (!certified && a.kc >= a.k && si[a.kc - 1] != INT_MAX) {
-      const double akc = (double)sd[a.kc - 1];
+    if (!certified && a.kc >= a.k && full_s) {
+      const double akc = akc_s;
       const double nx = a.xnorm ? (double)a.xnorm[q] : xnorm_s;
       const double ny = from_ordered_key(a.sc->max_ynorm_key);
       const double n = (double)a.acc_terms;

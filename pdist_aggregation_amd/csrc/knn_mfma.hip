@@ -71,11 +71,15 @@ constexpr int row_pitch() {
   return DM * Fmt<FMT>::esize + 16;
 }
 
-// MODE kModeSearch: the candidate search.  MODE kModeSample: the same
-// sweep over sampled row ranges, keeping only each lane's kSampleJ smallest
-// per-tile minima in registers (a threshold estimate, no lists).
+// MODE kModeSearch: the candidate search.  MODE kModeResearch: the same for
+// the query blocks / queries selected by qblock_active / query_active (a
+// separate instantiation, so profiles tell the two launches apart).  MODE
+// kModeSample: the same sweep over sampled row ranges, keeping only each
+// lane's kSampleJ smallest per-tile minima in registers (a threshold
+// estimate, no lists).
 constexpr int kModeSearch = 0;
 constexpr int kModeSample = 1;
+constexpr int kModeResearch = 2;
 
 template <int FMT, int DM, int QG, int MODE>
 __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
@@ -109,7 +113,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     split = w_item / a.n_qblocks;
     qb = w_item % a.n_qblocks;
   }
-  if (a.qblock_active && a.qblock_active[qb] == 0) return;  // re-search: block not selected
+  if (MODE == kModeResearch && a.qblock_active[qb] == 0) return;  // block not selected
   const int64_t q0 = (int64_t)qb * BQ;
   const int64_t ybeg = (int64_t)split * a.split_stride;
   const int64_t yend = ybeg + a.rows_per_split < a.n_t ? ybeg + a.rows_per_split : a.n_t;
@@ -129,7 +133,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   for (int g = 0; g < QG; ++g) {
     const int64_t q = q0 + wq0 + g * 32 + c;
     const bool qok = q < a.n_q;
-    qact[g] = qok && (!a.query_active || a.query_active[q] != 0);
+    qact[g] = qok && (MODE != kModeResearch || a.query_active[q] != 0);
     tau[g] = qact[g] ? FLT_MAX : -__builtin_huge_valf();
     if constexpr (FMT == kFmtF32) {
       const float* xrow = reinterpret_cast<const float*>(a.X) + q * a.ldx;
@@ -183,7 +187,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   for (int g = 0; g < QG; ++g) {
     cnt[g] = 0;
     const int64_t q = q0 + wq0 + g * 32 + c;
-    if (MODE == kModeSearch && qact[g])  // start from the best threshold known
+    if (MODE != kModeSample && qact[g])  // start from the best threshold known
       tau[g] = fminf(tau[g],
                      (float)from_ordered_key(load_agent(&a.tau[q])));
   }
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
         }
         // Publish this split's (slot_rank+1)-th smallest value (see select()).
         if (a.qslots && m > a.slot_rank)
-          __atomic_store_n(reinterpret_cast<unsigned*>(a.qslots) + (size_t)(q0 + ql) * a.n_splits +
+          __atomic_store_n(reinterpret_cast<unsigned*>(a.qslots) + (size_t)(q0 + ql) * a.slot_stride +
                                split,
                            __float_as_uint(sd[a.slot_rank]), __ATOMIC_RELAXED);
       }
@@ -439,18 +443,25 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
         if (qact[g]) {
           float t = (float)from_ordered_key(load_agent(&a.tau[q]));
           if (a.qslots) {
+            // Rows of slot_stride (a multiple of 8, padding +inf): eight
+            // loads in flight per round trip.
             float r[kSlotJ];
 #pragma unroll
             for (int j = 0; j < kSlotJ; ++j) r[j] = __builtin_huge_valf();
-            const float* sl = a.qslots + (size_t)q * a.n_splits;
-            for (int sidx = 0; sidx < a.n_splits; ++sidx) {
-              const float x = load_agent(sl + sidx);
+            const float* sl = a.qslots + (size_t)q * a.slot_stride;
+            for (int s0 = 0; s0 < a.slot_stride; s0 += 8) {
+              float x[8];
 #pragma unroll
-              for (int j = kSlotJ - 1; j > 0; --j) {
-                const float hi = r[j - 1] > x ? r[j - 1] : x;
-                r[j] = r[j] < hi ? r[j] : hi;
+              for (int u = 0; u < 8; ++u) x[u] = load_agent(sl + s0 + u);
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+#pragma unroll
+                for (int j = kSlotJ - 1; j > 0; --j) {
+                  const float hi = r[j - 1] > x[u] ? r[j - 1] : x[u];
+                  r[j] = r[j] < hi ? r[j] : hi;
+                }
+                r[0] = r[0] < x[u] ? r[0] : x[u];
               }
-              r[0] = r[0] < x ? r[0] : x;
             }
             t = t < r[kSlotJ - 1] ? t : r[kSlotJ - 1];
           }
@@ -549,7 +560,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     const int ql = wq0 + qq;
     const int64_t q = q0 + ql;
     if (q >= a.n_q) break;
-    if (a.query_active && a.query_active[q] == 0) continue;  // uniform across the wave
+    if (MODE == kModeResearch && a.query_active[q] == 0) continue;  // uniform across the wave
     const int n0 = cnt_s[2 * ql], n1 = cnt_s[2 * ql + 1];
     const int n = n0 + n1;
     if (n > 0) sort_query(ql, n0, n1);
@@ -565,7 +576,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
 
 template <int FMT, int DM, int QG, int MODE>
 void launch_one(const MfmaArgs& a, hipStream_t s) {
-  const size_t lds = MODE == kModeSearch ? (size_t)4 * a.cap * 8 : 0;
+  const size_t lds = MODE == kModeSample ? 0 : (size_t)4 * a.cap * 8;
   const unsigned blocks = (unsigned)((a.n_qblocks * a.n_splits + 7) / 8 * 8);
   hipLaunchKernelGGL((mfma_search_kernel<FMT, DM, QG, MODE>), dim3(blocks), dim3(kThreads), lds,
                      s, a);
@@ -634,5 +645,9 @@ static bool launch_mode(const MfmaArgs& a, hipStream_t s) {
 bool launch_mfma_search(const MfmaArgs& a, hipStream_t s) { return launch_mode<kModeSearch>(a, s); }
 
 bool launch_mfma_sample(const MfmaArgs& a, hipStream_t s) { return launch_mode<kModeSample>(a, s); }
+
+bool launch_mfma_research(const MfmaArgs& a, hipStream_t s) {
+  return a.qblock_active && a.query_active && launch_mode<kModeResearch>(a, s);
+}
 
 }  // namespace knn

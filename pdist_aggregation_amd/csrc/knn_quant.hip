@@ -16,6 +16,7 @@ namespace knn {
 namespace {
 
 constexpr int kSampleRowsPerBlock = 256;
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(kThreads) void column_sum_kernel(const float* __restrict__ Y,
                                                               int64_t n, int d,
@@ -200,12 +201,240 @@ __global__ __launch_bounds__(kThreads) void quantize_fp8_kernel(
   }
 }
 
+// ---- vectorised forms: d % 4 == 0, 16-byte aligned rows --------------------
+// LPR lanes cover one row of dm = 4 * LPR features (4 per lane, float4 loads),
+// so a wave holds 64 / LPR rows at once and each lane keeps kVecRows rows'
+// loads in flight.
+constexpr int kVecRows = 4;
+
+__global__ __launch_bounds__(kThreads) void column_sum_vec_kernel(const float* __restrict__ Y,
+                                                                  int64_t n, int d,
+                                                                  float* __restrict__ mu_sum) {
+  __shared__ f4 part[kThreads];
+  const int groups = d >> 2;                 // float4 columns (<= 64)
+  const int lanes_r = kThreads / groups;     // rows summed in parallel
+  const int cg = threadIdx.x % groups, rl = threadIdx.x / groups;
+  const int64_t r0 = (int64_t)blockIdx.x * kSampleRowsPerBlock;
+  const int64_t r1 = r0 + kSampleRowsPerBlock < n ? r0 + kSampleRowsPerBlock : n;
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  if (rl < lanes_r)
+    for (int64_t r = r0 + rl; r < r1; r += lanes_r) {
+      const f4 v = *reinterpret_cast<const f4*>(Y + r * d + 4 * cg);
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  if (rl == 0) {
+    for (int i = 1; i < lanes_r; ++i) {
+      const f4 v = part[i * groups + cg];
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
+    atomicAdd(&mu_sum[4 * cg + 0], acc.x);
+    atomicAdd(&mu_sum[4 * cg + 1], acc.y);
+    atomicAdd(&mu_sum[4 * cg + 2], acc.z);
+    atomicAdd(&mu_sum[4 * cg + 3], acc.w);
+  }
+}
+
+// Block-wide maxima of the row norms and error bounds -> sc.
+__device__ inline void track_maxima(float best_n, float best_e, DeviceScalars* sc) {
+  __shared__ float wn[kThreads / 64];
+  __shared__ float we[kThreads / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    best_n = fmaxf(best_n, __shfl_xor(best_n, o, 64));
+    best_e = fmaxf(best_e, __shfl_xor(best_e, o, 64));
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    wn[w] = best_n;
+    we[w] = best_e;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float mn = wn[0], me = we[0];
+    for (int i = 1; i < kThreads / 64; ++i) {
+      mn = fmaxf(mn, wn[i]);
+      me = fmaxf(me, we[i]);
+    }
+    atomicMax(&sc->max_ynorm_key, ordered_key((double)mn));
+    atomicMax(&sc->max_yerr_key, ordered_key((double)me));
+  }
+}
+
+template <int LPR>
+__global__ __launch_bounds__(kThreads) void quantize_bf16_vec_kernel(
+    const float* __restrict__ src, int64_t n, int d, const float* __restrict__ mu,
+    __bf16* __restrict__ dst, float* __restrict__ norm, float* __restrict__ err,
+    DeviceScalars* sc, int track_max) {
+  constexpr int DM = 4 * LPR;
+  constexpr int RPB = (kThreads / LPR);  // rows per block and pass
+  const int l = threadIdx.x % LPR, rsub = threadIdx.x / LPR;
+  const int f0 = 4 * l;
+  const bool fok = f0 < d;
+  const f4 mu4 = fok ? *reinterpret_cast<const f4*>(mu + f0) : f4{0.f, 0.f, 0.f, 0.f};
+  float best_n = 0.f, best_e = 0.f;
+  for (int64_t base = (int64_t)blockIdx.x * RPB * kVecRows; base < n;
+       base += (int64_t)gridDim.x * RPB * kVecRows) {
+    f4 x[kVecRows];
+#pragma unroll
+    for (int u = 0; u < kVecRows; ++u) {
+      const int64_t r = base + u * RPB + rsub;
+      x[u] = (fok && r < n) ? *reinterpret_cast<const f4*>(src + r * d + f0)
+                            : f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < kVecRows; ++u) {
+      const int64_t r = base + u * RPB + rsub;
+      const float xv[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+      const float mv[4] = {mu4.x, mu4.y, mu4.z, mu4.w};
+      __bf16 qb[4];
+      float ns = 0.f;
+      double es = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        qb[i] = fok ? (__bf16)(xv[i] - mv[i]) : (__bf16)0.f;
+        const float q = (float)qb[i];
+        ns = fmaf(q, q, ns);
+        const double e = fok ? ((double)xv[i] - (double)mv[i]) - (double)q : 0.0;
+        es = fma(e, e, es);
+      }
+#pragma unroll
+      for (int o = LPR / 2; o > 0; o >>= 1) {
+        ns += __shfl_xor(ns, o, 64);
+        es += __shfl_xor(es, o, 64);
+      }
+      if (r < n) {
+        unsigned short h[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[i] = __builtin_bit_cast(unsigned short, qb[i]);
+        const uint2 packed = {(unsigned)h[0] | ((unsigned)h[1] << 16),
+                              (unsigned)h[2] | ((unsigned)h[3] << 16)};
+        *reinterpret_cast<uint2*>(dst + r * DM + f0) = packed;
+        const float eb = __uint_as_float(__float_as_uint((float)sqrt(es)) + 1u);  // next float up
+        if (l == 0) {
+          norm[r] = ns;
+          if (err) err[r] = eb;
+        }
+        best_n = fmaxf(best_n, ns);
+        best_e = fmaxf(best_e, eb);
+      }
+    }
+  }
+  if (track_max) track_maxima(best_n, best_e, sc);
+}
+
+template <int LPR>
+__global__ __launch_bounds__(kThreads) void quantize_fp8_vec_kernel(
+    const float* __restrict__ src, int64_t n, int d, const float* __restrict__ mu,
+    unsigned char* __restrict__ dst, unsigned char* __restrict__ scale, float* __restrict__ norm,
+    float* __restrict__ err, DeviceScalars* sc, int track_max) {
+  constexpr int DM = 4 * LPR;
+  constexpr int RPB = (kThreads / LPR);
+  const int l = threadIdx.x % LPR, rsub = threadIdx.x / LPR;
+  const int f0 = 4 * l;
+  const bool fok = f0 < d;
+  const f4 mu4 = fok ? *reinterpret_cast<const f4*>(mu + f0) : f4{0.f, 0.f, 0.f, 0.f};
+  float best_n = 0.f, best_e = 0.f;
+  for (int64_t base = (int64_t)blockIdx.x * RPB * kVecRows; base < n;
+       base += (int64_t)gridDim.x * RPB * kVecRows) {
+    f4 x[kVecRows];
+#pragma unroll
+    for (int u = 0; u < kVecRows; ++u) {
+      const int64_t r = base + u * RPB + rsub;
+      x[u] = (fok && r < n) ? *reinterpret_cast<const f4*>(src + r * d + f0)
+                            : f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < kVecRows; ++u) {
+      const int64_t r = base + u * RPB + rsub;
+      const float xv[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+      const float mv[4] = {mu4.x, mu4.y, mu4.z, mu4.w};
+      float cen[4], amax = 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        cen[i] = fok ? xv[i] - mv[i] : 0.f;
+        amax = fmaxf(amax, fabsf(cen[i]));
+      }
+#pragma unroll
+      for (int o = LPR / 2; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+      int e = 0;
+      if (amax > 0.f) {
+        int E;
+        const float fr = frexpf(amax / 448.f, &E);  // amax/448 = fr * 2^E
+        e = fr > 0.5f ? E : E - 1;                   // smallest e with amax/448 <= 2^e
+        e = e < -120 ? -120 : (e > 120 ? 120 : e);
+      }
+      unsigned word = 0;
+      float ns = 0.f;
+      double es = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const unsigned b = fok ? encode_e4m3(ldexpf(cen[i], -e)) : 0u;
+        word |= b << (8 * i);
+        const float q = ldexpf(decode_e4m3(b), e);
+        ns = fmaf(q, q, ns);
+        const double ed = fok ? ((double)xv[i] - (double)mv[i]) - (double)q : 0.0;
+        es = fma(ed, ed, es);
+      }
+#pragma unroll
+      for (int o = LPR / 2; o > 0; o >>= 1) {
+        ns += __shfl_xor(ns, o, 64);
+        es += __shfl_xor(es, o, 64);
+      }
+      if (r < n) {
+        *reinterpret_cast<unsigned*>(dst + r * DM + f0) = word;
+        const float eb = __uint_as_float(__float_as_uint((float)sqrt(es)) + 1u);
+        if (l == 0) {
+          norm[r] = ns;
+          scale[r] = (unsigned char)(e + 127);  // E8M0
+          if (err) err[r] = eb;
+        }
+        best_n = fmaxf(best_n, ns);
+        best_e = fmaxf(best_e, eb);
+      }
+    }
+  }
+  if (track_max) track_maxima(best_n, best_e, sc);
+}
+
+bool vec_ok(const float* src, int d, int dm) {
+  return (d & 3) == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0 && dm >= 32 && dm <= 256;
+}
+
+int64_t vec_blocks(int64_t n, int dm) {
+  const int64_t rows_per_block = (int64_t)(kThreads / (dm / 4)) * kVecRows;
+  const int64_t b = ceil_div(n, rows_per_block);
+  return b < 8192 ? b : 8192;
+}
+
 }  // namespace
 
 void launch_quantize_fp8(const float* src, int64_t n, int d, int dm, const float* mu, void* dst,
                          unsigned char* scale, float* norm, float* err, DeviceScalars* sc,
                          bool track_max, hipStream_t s) {
   if (n <= 0) return;
+  if (vec_ok(src, d, dm)) {
+    auto* out = reinterpret_cast<unsigned char*>(dst);
+    const dim3 g((unsigned)vec_blocks(n, dm)), b(kThreads);
+    const int tm = track_max ? 1 : 0;
+    if (dm == 64)
+      hipLaunchKernelGGL(quantize_fp8_vec_kernel<16>, g, b, 0, s, src, n, d, mu, out, scale, norm,
+                         err, sc, tm);
+    else if (dm == 128)
+      hipLaunchKernelGGL(quantize_fp8_vec_kernel<32>, g, b, 0, s, src, n, d, mu, out, scale, norm,
+                         err, sc, tm);
+    else if (dm == 256)
+      hipLaunchKernelGGL(quantize_fp8_vec_kernel<64>, g, b, 0, s, src, n, d, mu, out, scale, norm,
+                         err, sc, tm);
+    if (dm == 64 || dm == 128 || dm == 256) return;
+  }
   const int64_t blocks = ceil_div(n, 4) < 16384 ? ceil_div(n, 4) : 16384;
   hipLaunchKernelGGL(quantize_fp8_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, src, n, d,
                      dm, mu, reinterpret_cast<unsigned char*>(dst), scale, norm, err, sc,
@@ -216,8 +445,12 @@ void launch_column_mean(const float* Y, int64_t n_sample, int d, float* mu_sum, 
                         hipStream_t s) {
   hipLaunchKernelGGL(column_zero_kernel, dim3(1), dim3(kThreads), 0, s, mu_sum, d);
   const int64_t blocks = ceil_div(n_sample, kSampleRowsPerBlock);
-  hipLaunchKernelGGL(column_sum_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, Y, n_sample,
-                     d, mu_sum);
+  if ((d & 3) == 0 && d <= 4 * 64 && (reinterpret_cast<uintptr_t>(Y) & 15) == 0)
+    hipLaunchKernelGGL(column_sum_vec_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, Y,
+                       n_sample, d, mu_sum);
+  else
+    hipLaunchKernelGGL(column_sum_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, Y,
+                       n_sample, d, mu_sum);
   hipLaunchKernelGGL(column_scale_kernel, dim3(1), dim3(kThreads), 0, s, mu_sum, d,
                      1.0f / (float)n_sample, mu);
 }
@@ -226,6 +459,24 @@ void launch_quantize_bf16(const float* src, int64_t n, int d, int dm, const floa
                           float* norm, float* err, DeviceScalars* sc, bool track_max,
                           hipStream_t s) {
   if (n <= 0) return;
+  if (vec_ok(src, d, dm)) {
+    auto* out = reinterpret_cast<__bf16*>(dst);
+    const dim3 g((unsigned)vec_blocks(n, dm)), b(kThreads);
+    const int tm = track_max ? 1 : 0;
+    if (dm == 32)
+      hipLaunchKernelGGL(quantize_bf16_vec_kernel<8>, g, b, 0, s, src, n, d, mu, out, norm, err,
+                         sc, tm);
+    else if (dm == 64)
+      hipLaunchKernelGGL(quantize_bf16_vec_kernel<16>, g, b, 0, s, src, n, d, mu, out, norm, err,
+                         sc, tm);
+    else if (dm == 128)
+      hipLaunchKernelGGL(quantize_bf16_vec_kernel<32>, g, b, 0, s, src, n, d, mu, out, norm, err,
+                         sc, tm);
+    else if (dm == 256)
+      hipLaunchKernelGGL(quantize_bf16_vec_kernel<64>, g, b, 0, s, src, n, d, mu, out, norm, err,
+                         sc, tm);
+    if (dm == 32 || dm == 64 || dm == 128 || dm == 256) return;
+  }
   const int64_t blocks = ceil_div(n, 4) < 16384 ? ceil_div(n, 4) : 16384;
   hipLaunchKernelGGL(quantize_bf16_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, src, n, d,
                      dm, mu, reinterpret_cast<__bf16*>(dst), norm, err, sc, track_max ? 1 : 0);
