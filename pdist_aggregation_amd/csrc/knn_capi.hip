@@ -300,7 +300,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     return at;
   };
   p.off_scalars = take(sizeof(knn::DeviceScalars));
-  p.off_ynorm = take((size_t)n_train * celem);
+  p.off_ynorm = take((size_t)(n_train + knn::kNormPad) * celem);
   p.off_tau = take((size_t)nq1 * sizeof(unsigned long long));
   const size_t ncand = (size_t)p.n_qblocks * p.n_splits * bq * p.cap;
   p.off_cand_d = take(ncand * celem);
@@ -527,6 +527,9 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       ma.unit_roundoff = 0x1p-24;
       ma.acc_terms = 2 * p.dm + 4;
     }
+    // |y|^2 = +inf past the last train row (the kernels' partial last tile)
+    knn::launch_fill_u32(reinterpret_cast<unsigned*>(ynorm) + n_train, knn::kNormPad,
+                         0x7f800000u, stream);
     // 2b. sampled threshold: minima over 1/32 of the rows -> guess -> tau
     if (p.guess) {
       auto* failed = reinterpret_cast<int*>(ws + p.off_failed);

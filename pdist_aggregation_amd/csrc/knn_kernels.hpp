@@ -54,6 +54,9 @@ constexpr int kFmtF32 = 0;   // v_mfma_f32_32x32x2_f32 on the caller's rows
 constexpr int kFmtBF16 = 1;  // v_mfma_f32_32x32x16_bf16 on quantised rows
 constexpr int kFmtFP8 = 2;   // v_mfma_scale_f32_32x32x64_f8f6f4, E4M3 + row scales
 constexpr int kMfmaRowStep = 32;
+// +inf entries after the n_t row norms the MFMA kernels read: rows of the
+// last (partial) tile past n_t get |y|^2 = +inf and are never selected.
+constexpr int kNormPad = 128;
 constexpr int kMfmaMaxCap = 512;  // per-query list capacity (wave sort in LDS)
 // A query's list is two halves of cap/2, one per owning lane; each half must
 // hold ceil(k'/2) survivors plus one step's 16 appends: cap >= k' + 34.

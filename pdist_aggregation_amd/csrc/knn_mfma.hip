@@ -43,6 +43,8 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBY = 32;  // train rows per step (one MFMA row block)
 
+__device__ inline float fmin_ieee(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+
 template <int FMT>
 struct Fmt;
 template <>
@@ -200,41 +202,46 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     i32x4 v[CPT];
     float n;
     int e;
-    int64_t t;
   };
   // Raw buffer loads over this split's rows: rows past the split read as
   // zero (range-checked by the hardware), so the loads carry no branches and
-  // the compiler can wait for exactly the loads it needs.
-  const size_t row_bytes = (size_t)a.ldy * F::esize;
+  // the compiler can wait for exactly the loads it needs.  The tile base is
+  // added to the per-lane offsets (the range check covers the vector offset;
+  // the scalar offset is kept at 0).
+  // Splits hold whole tiles except the last one of the train set, whose rows
+  // past n_t take |y|^2 = +inf from the padding after the norms (kNormPad
+  // entries): their values are +inf, never selected, with no mask here.  The
+  // norms of look-ahead tiles past a split are loaded but never used.
+  const int row_bytes = a.ldy * F::esize;
   const int64_t nrows = yend - ybeg;
   const __amdgpu_buffer_rsrc_t yrs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<unsigned char*>(reinterpret_cast<const unsigned char*>(a.Y)) + ybeg * row_bytes,
+      const_cast<unsigned char*>(reinterpret_cast<const unsigned char*>(a.Y)) +
+          ybeg * (int64_t)row_bytes,
       (short)0, (int)(nrows * row_bytes), 0x00020000);
   const __amdgpu_buffer_rsrc_t nrs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float*>(a.ynorm + ybeg), (short)0, (int)(nrows * 4), 0x00020000);
+      const_cast<float*>(a.ynorm + ybeg), (short)0, (int)((nrows + 3 * kBY) * 4), 0x00020000);
   __amdgpu_buffer_rsrc_t ers = nrs;
   if constexpr (FMT == kFmtFP8)
     ers = __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned char*>(a.yscale + ybeg), (short)0,
                                             (int)nrows, 0x00020000);
-  auto load_tile = [&](Staged& st, int64_t t) {  // tile t of this split
+  int ld_off[CPT];
 #pragma unroll
-    for (int i = 0; i < CPT; ++i) {
-      const int ch = tid + i * kThreads;
-      const int row = ch / (DM * F::esize / 16);
-      const int col = ch % (DM * F::esize / 16);
-      const int f = col * (16 / F::esize);
-      const int off = (int)((t * kBY + row) * row_bytes) + f * F::esize;
-      st.v[i] = __builtin_amdgcn_raw_buffer_load_b128(yrs, off, 0, 0);
-    }
-    const int r = (int)(t * kBY) + (tid & (kBY - 1));
-    st.n = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(nrs, r * 4, 0, 0));
+  for (int i = 0; i < CPT; ++i) {
+    const int ch = tid + i * kThreads;
+    ld_off[i] = (ch / (DM * F::esize / 16)) * row_bytes + (ch % (DM * F::esize / 16)) * 16;
+  }
+  const int rl = tid & (kBY - 1);
+  auto load_tile = [&](Staged& st, int t) {  // tile t of this split
+    const int tb = t * kBY;
+#pragma unroll
+    for (int i = 0; i < CPT; ++i)
+      st.v[i] = __builtin_amdgcn_raw_buffer_load_b128(yrs, ld_off[i] + tb * row_bytes, 0, 0);
+    st.n = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(nrs, (tb + rl) * 4, 0, 0));
     st.e = 127;
-    if constexpr (FMT == kFmtFP8) st.e = (int)__builtin_amdgcn_raw_buffer_load_b8(ers, r, 0, 0);
-    st.t = t;
+    if constexpr (FMT == kFmtFP8) st.e = (int)__builtin_amdgcn_raw_buffer_load_b8(ers, tb + rl, 0, 0);
   };
   // Masking happens here, not at the load, so nothing waits on the data
-  // before it is needed: features past d read as 0, rows past the split get
-  // |y|^2 = +inf (never selected).
+  // before it is needed: float32 features past d read as 0.
   auto store_tile = [&](const Staged& st, int buf) {
 #pragma unroll
     for (int i = 0; i < CPT; ++i) {
@@ -250,9 +257,8 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     }
     // Every thread holds the norm of row (tid % 32); the 8 copies write the
     // same value, without a branch.
-    const int r = (int)(st.t * kBY) + (tid & (kBY - 1));
-    yn_s[buf][tid & (kBY - 1)] = r < nrows ? st.n : __builtin_huge_valf();
-    if constexpr (FMT == kFmtFP8) ye_s[buf][tid & (kBY - 1)] = r < nrows ? st.e : 127;
+    yn_s[buf][rl] = st.n;
+    if constexpr (FMT == kFmtFP8) ye_s[buf][rl] = st.e;
   };
 
   // ---- one 32-row tile on the matrix cores ------------------------------------
@@ -398,20 +404,23 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     bool overflow = false;
 #pragma unroll
     for (int g = 0; g < QG; ++g) {
+      // IEEE minimum (NaN-propagating, v_minimum3_f32 on gfx950: no operand
+      // canonicalisation).  A NaN makes a quad "not above tau", so its
+      // values are tested one by one and a NaN hides no other row.
       float m4[4];
 #pragma unroll
       for (int r4 = 0; r4 < 4; ++r4)
-        m4[r4] = fminf(fminf(acc[g][4 * r4], acc[g][4 * r4 + 1]),
-                       fminf(acc[g][4 * r4 + 2], acc[g][4 * r4 + 3]));
-      const float m = fminf(fminf(m4[0], m4[1]), fminf(m4[2], m4[3]));
-      if (__any(m <= tau[g])) {
+        m4[r4] = fmin_ieee(fmin_ieee(acc[g][4 * r4], acc[g][4 * r4 + 1]),
+                           fmin_ieee(acc[g][4 * r4 + 2], acc[g][4 * r4 + 3]));
+      const float m = fmin_ieee(fmin_ieee(m4[0], m4[1]), fmin_ieee(m4[2], m4[3]));
+      if (__any(!(m > tau[g]))) {
         const int ql = wq0 + g * 32 + c;
         const int base = (ql * a.cap + h * half) * 4;
         const int yb = (int)y0 + h * 4;
         int n = cnt[g];
 #pragma unroll
         for (int r4 = 0; r4 < 4; ++r4) {
-          if (m4[r4] <= tau[g]) {
+          if (!(m4[r4] > tau[g])) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
               const float v = acc[g][4 * r4 + e];
@@ -481,7 +490,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   // the tile t+2 loads issued at the top of the same step.  Offsets stay
   // below 2^31: a split holds at most 1 GiB of rows, plus the 95 rows of
   // look-ahead.
-  const int64_t ntiles = yend > ybeg ? (yend - ybeg + kBY - 1) / kBY : 0;
+  const int ntiles = yend > ybeg ? (int)((yend - ybeg + kBY - 1) / kBY) : 0;
   // `sel(acc, y0, step)` consumes one finished tile.
   auto sweep = [&](auto&& sel) {
     Staged stA, stB;
@@ -492,24 +501,24 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     __syncthreads();
     // One step; `cur` is the LDS buffer of tile t, `ld` the free register
     // set, `st` the set holding tile t+1.
-    auto one_step = [&](int64_t t, f32x16(&accC)[QG], f32x16(&accS)[QG], Staged& ld,
+    auto one_step = [&](int t, f32x16(&accC)[QG], f32x16(&accS)[QG], Staged& ld,
                         Staged& st) {
       const int cur = (int)(t & 1);
       load_tile(ld, t + 2);
       compute(accC, cur);
       store_tile(st, cur ^ 1);
-      if (t > 0) sel(accS, ybeg + (t - 1) * kBY, (int)(t - 1));
+      if (t > 0) sel(accS, ybeg + (int64_t)(t - 1) * kBY, t - 1);
       __syncthreads();
     };
-    for (int64_t t = 0; t < ntiles; t += 2) {
+    for (int t = 0; t < ntiles; t += 2) {
       one_step(t, accA, accB, stA, stB);
       if (t + 1 < ntiles) one_step(t + 1, accB, accA, stB, stA);
     }
     if (ntiles > 0) {
       if ((ntiles - 1) & 1)
-        sel(accB, ybeg + (ntiles - 1) * kBY, (int)(ntiles - 1));
+        sel(accB, ybeg + (int64_t)(ntiles - 1) * kBY, ntiles - 1);
       else
-        sel(accA, ybeg + (ntiles - 1) * kBY, (int)(ntiles - 1));
+        sel(accA, ybeg + (int64_t)(ntiles - 1) * kBY, ntiles - 1);
     }
   };
 
@@ -525,9 +534,9 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
     sweep([&](f32x16(&acc)[QG], int64_t, int) {
 #pragma unroll
       for (int g = 0; g < QG; ++g) {
-        float m = acc[g][0];
+        float m = acc[g][0];  // a NaN value drops this step's minimum (m < b fails)
 #pragma unroll
-        for (int i = 1; i < 16; ++i) m = fminf(m, acc[g][i]);
+        for (int i = 1; i < 16; ++i) m = fmin_ieee(m, acc[g][i]);
         if (__any(qact[g] && m < best[g][kSampleJ - 1])) {
 #pragma unroll
           for (int i = kSampleJ - 1; i > 0; --i)
