@@ -90,6 +90,38 @@ class Runner:
         return {key: float(np.mean(v)) for key, v in acc.items()}, st.as_dict()
 
 
+def profiled_traffic(precision, workload):
+    """HBM bytes per launch of the search kernel, from the rocprofv3 counter
+    summary committed under profiles/ (FETCH_SIZE + WRITE_SIZE, each scaled by
+    the same counter on a copy kernel of known size); None if not profiled."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    prof = os.path.join(here, "profiles")
+    if not os.path.isdir(prof):
+        return None
+    for name in sorted(os.listdir(prof)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(prof, name)) as f:
+            for line in f:
+                if not line.startswith("bench-traffic:"):
+                    continue
+                kv = dict(t.split("=", 1) for t in line.split()[1:])
+                if kv.get("precision") == precision and kv.get("workload") == workload:
+                    return {"bytes": float(kv["bytes_per_launch"]),
+                            "fetch_bytes": float(kv["fetch_bytes"]),
+                            "write_bytes": float(kv["write_bytes"]), "source": f"profiles/{name}"}
+    return None
+
+
+def algo_bytes(args, precision):
+    """Bytes a launch must move at least: the train rows in the search format
+    (+ their norms), the queries, and the k' lists it writes."""
+    esize = {"fp32": 4, "bf16": 2, "fp8": 1}[precision]
+    kc = args.k + max(8, args.k // 8) if precision == "fp32" else 4 * args.k
+    return (args.nt * (args.dim * esize + 4) + args.nq * args.dim * esize
+            + args.nq * kc * 8)
+
+
 def recall(idx, ref_idx):
     hits = 0
     for a, b in zip(idx, ref_idx):
@@ -191,7 +223,9 @@ def main():
             print(f"[bench] {prec}: unavailable ({e})", file=sys.stderr, flush=True)
             continue
         ms, wall = run.timed(args.steps, args.warmup)
-        stages, st = run.stage_times(max(3, min(args.steps, 10)))
+        # Stage durations: HIP events the library records on its stream around
+        # each stage, averaged over another `steps` calls.
+        stages, st = run.stage_times(max(3, args.steps))
         res = {"queries_per_s": args.nq / (ms * 1e-3), "ms_per_step": ms, "wall_s": wall,
                "stages_ms": stages, "kernel_path": _capi.KERNEL_PATH.get(st["kernel_path"]),
                "k_candidates": st["k_candidates"], "n_splits": st["n_splits"],
@@ -214,6 +248,8 @@ def main():
     h = ok[head]
     search_s = h["stages_ms"]["search_ms"] * 1e-3
     achieved = flops / search_s / 1e12
+    workload = f"{args.nq:.0e}x{args.nt:.0e}x{args.dim}".replace("+0", "").replace("e0", "e")
+    traffic = profiled_traffic(head, workload)
     line = {
         "metric": METRIC,
         "value": h["queries_per_s"],
@@ -233,8 +269,11 @@ def main():
                    "n_queries": args.nq, "n_train": args.nt, "dim": args.dim, "k": args.k},
         "roofline": {"bound": "mfma", "kernel": "fused distance + candidate search",
                      "achieved": achieved, "peak": PEAK[head], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[head], "traffic": None,
+                     "frac": achieved / PEAK[head],
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_detail": traffic,
                      "algorithmic_flops_per_launch": flops,
+                     "algorithmic_bytes_per_launch": algo_bytes(args, head),
                      "kernel_ms": h["stages_ms"]["search_ms"]},
         "recall_at_k": h.get("recall_at_k"),
         "results": results,
