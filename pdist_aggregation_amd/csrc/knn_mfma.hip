@@ -84,7 +84,7 @@ constexpr int kModeSample = 1;
 constexpr int kModeResearch = 2;
 
 template <int FMT, int DM, int QG, int MODE>
-__global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
+__global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_kernel(MfmaArgs a) {
   using F = Fmt<FMT>;
   constexpr int PITCH = row_pitch<FMT, DM>();
   constexpr int WQ = QG * 32;           // queries per wave
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   // ---- main sweep --------------------------------------------------------
   // Step t (tile t in LDS buffer t%2, tile t+1 in flight in registers):
   //   issue the loads of tile t+2; MFMAs of tile t; write tile t+1 to LDS;
-  //   select tile t-1 (its VALU work overlaps the MFMAs); barrier.
+  //   select tile t; barrier.
   // The loads are issued unconditionally (tiles past the split read as zero
   // through the buffer range check), so every step issues the same number of
   // them and writing tile t+1 waits only for tile t+1's own loads, never for
@@ -492,33 +492,32 @@ __global__ __launch_bounds__(kThreads, 2) void mfma_search_kernel(MfmaArgs a) {
   // look-ahead.
   const int ntiles = yend > ybeg ? (int)((yend - ybeg + kBY - 1) / kBY) : 0;
   // `sel(acc, y0, step)` consumes one finished tile.
+  // One accumulator set: each tile is selected right after its own MFMAs.
+  // Keeping a second set to select tile t-1 during tile t's MFMAs costs
+  // 16*QG registers, which is the difference between 2 and 3 resident waves
+  // per SIMD at bf16 d=128; with 3 the other waves fill the wait for the
+  // MFMA results (measured: 3.15 -> 2.48 ms at 1e4 x 1e6 x 128).
+  // `sel(acc, y0, step)` consumes one finished tile.
   auto sweep = [&](auto&& sel) {
     Staged stA, stB;
-    f32x16 accA[QG], accB[QG];
+    f32x16 acc[QG];
     load_tile(stA, 0);
     store_tile(stA, 0);
     load_tile(stB, 1);
     __syncthreads();
     // One step; `cur` is the LDS buffer of tile t, `ld` the free register
     // set, `st` the set holding tile t+1.
-    auto one_step = [&](int t, f32x16(&accC)[QG], f32x16(&accS)[QG], Staged& ld,
-                        Staged& st) {
-      const int cur = (int)(t & 1);
+    auto one_step = [&](int t, Staged& ld, Staged& st) {
+      const int cur = t & 1;
       load_tile(ld, t + 2);
-      compute(accC, cur);
+      compute(acc, cur);
       store_tile(st, cur ^ 1);
-      if (t > 0) sel(accS, ybeg + (int64_t)(t - 1) * kBY, t - 1);
+      sel(acc, ybeg + (int64_t)t * kBY, t);
       __syncthreads();
     };
     for (int t = 0; t < ntiles; t += 2) {
-      one_step(t, accA, accB, stA, stB);
-      if (t + 1 < ntiles) one_step(t + 1, accB, accA, stB, stA);
-    }
-    if (ntiles > 0) {
-      if ((ntiles - 1) & 1)
-        sel(accB, ybeg + (int64_t)(ntiles - 1) * kBY, ntiles - 1);
-      else
-        sel(accA, ybeg + (int64_t)(ntiles - 1) * kBY, ntiles - 1);
+      one_step(t, stA, stB);
+      if (t + 1 < ntiles) one_step(t + 1, stB, stA);
     }
   };
 
