@@ -97,6 +97,17 @@ __global__ __launch_bounds__(kThreads) void row_norms_kernel(const T* __restrict
   }
 }
 
+template <class T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
 // Candidates left after the threshold filter up to which the k' best are
 // found by ranking (each thread counts the entries ahead of its own, no
 // barriers); more than that are bitonic-sorted.
@@ -118,6 +129,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   int* ei = reinterpret_cast<int*>(dyn_smem + o_ei);
   __shared__ double xnorm_s, akc_s;
   __shared__ int n_s, full_s;
+  __shared__ T cut_s;
 
   const int64_t q = blockIdx.x;
   if (a.only_queries && a.only_queries[q] == 0) return;  // re-search merge: not selected
@@ -129,12 +141,45 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   // or every entry passed the sampled guess): entries above it are dropped
   // here.  Without k' entries at or below it the query fails the check below.
   const T tq = a.tau ? (T)from_ordered_key(a.tau[q]) : pos_inf<T>();
+  // Each split's list is sorted, so the k'-th smallest of the first r
+  // entries of every split (r * n_splits >= k') bounds the k'-th value from
+  // above: entries past it are dropped too, which leaves a few more than k'.
+  const int r = a.kc < (a.kc + a.n_splits - 1) / a.n_splits + 1 ? a.kc
+                                                                 : (a.kc + a.n_splits - 1) / a.n_splits + 1;
+  if (tid == 0) {
+    n_s = 0;
+    cut_s = tq;
+  }
+  __syncthreads();
+  for (int e = tid; e < a.n_splits * r; e += kThreads) {
+    const int at = (e / r) * a.kc + e % r;
+    const int id = pi[at];
+    const T v = pd[at];
+    if (id >= 0 && id < a.n_t && v <= tq) {
+      const int h = atomicAdd(&n_s, 1);
+      sd[h] = v;
+      si[h] = id;
+    }
+  }
+  __syncthreads();
+  const int nh = n_s;
+  if (nh >= a.kc) {
+    for (int e = tid; e < nh; e += kThreads) {
+      const T v = sd[e];
+      const int id = si[e];
+      int rank = 0;
+      for (int j = 0; j < nh; ++j) rank += key_less(sd[j], si[j], v, id) ? 1 : 0;
+      if (rank == a.kc - 1) cut_s = v;  // one thread: keys are distinct
+    }
+  }
+  __syncthreads();
+  const T cut = cut_s;
   if (tid == 0) n_s = 0;
   __syncthreads();
   for (int e = tid; e < total; e += kThreads) {
     const int id = pi[e];
     const T v = pd[e];
-    if (id >= 0 && id < a.n_t && v <= tq) {  // never index past Y
+    if (id >= 0 && id < a.n_t && v <= cut) {  // never index past Y
       const int at = atomicAdd(&n_s, 1);
       sd[at] = v;
       si[at] = id;
@@ -200,6 +245,10 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   // reduction over the tpc lanes.
   const T* xr = a.X + q * a.d;
   const int tpc = KP >= kThreads ? 1 : (kThreads / KP < 64 ? kThreads / KP : 64);  // 2^i <= 64
+  using V4 = typename Vec4<T>::type;
+  const bool vec4 = (a.d & 3) == 0 &&
+                    ((reinterpret_cast<uintptr_t>(a.X) | reinterpret_cast<uintptr_t>(a.Y)) &
+                     (4 * sizeof(T) - 1)) == 0;
   const int per_pass = kThreads / tpc;
   for (int j0 = 0; j0 < KP; j0 += per_pass) {
     const int j = j0 + tid / tpc, c = tid % tpc;
@@ -207,9 +256,24 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     double s = 0.0;
     if (id != INT_MAX) {
       const T* yr = a.Y + (int64_t)id * a.d;
-      for (int p = c; p < a.d; p += tpc) {
-        const double df = (double)xr[p] - (double)yr[p];
-        s = fma(df, df, s);
+      if (vec4) {  // 16-byte loads, every one of them in flight at once
+        const int d4 = a.d >> 2;
+#pragma unroll 8
+        for (int p = c; p < d4; p += tpc) {
+          const V4 xv = reinterpret_cast<const V4*>(xr)[p];
+          const V4 yv = reinterpret_cast<const V4*>(yr)[p];
+          const double d0 = (double)xv.x - (double)yv.x, d1 = (double)xv.y - (double)yv.y;
+          const double d2 = (double)xv.z - (double)yv.z, d3 = (double)xv.w - (double)yv.w;
+          s = fma(d0, d0, s);
+          s = fma(d1, d1, s);
+          s = fma(d2, d2, s);
+          s = fma(d3, d3, s);
+        }
+      } else {
+        for (int p = c; p < a.d; p += tpc) {
+          const double df = (double)xr[p] - (double)yr[p];
+          s = fma(df, df, s);
+        }
       }
     }
     for (int o = tpc >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
