@@ -317,7 +317,8 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   }
   if (p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8) {
     const size_t qe = p.path == kPathMfmaBF16 ? 2 : 1;
-    p.off_mu = take(2 * (size_t)p.dm * sizeof(float));
+    p.off_mu = take((1 + (size_t)knn::column_mean_blocks(std::min<int64_t>(n_train, 65536))) *
+                    p.dm * sizeof(float));
     p.off_xq = take((size_t)nq1 * p.dm * qe);
     p.off_yq = take((size_t)n_train * p.dm * qe);
     p.off_xnorm = take((size_t)nq1 * sizeof(float));

@@ -126,9 +126,11 @@ int mfma_blocks_per_cu(int fmt, int dm, int cap);
 // ---------------------------------------------------------------------------
 // Low-precision preparation (knn_quant.hip)
 // ---------------------------------------------------------------------------
-// mu[f] = mean of feature f over the first n_sample rows of Y (f < d).
-void launch_column_mean(const float* Y, int64_t n_sample, int d, float* mu_sum, float* mu,
+// mu[f] = mean of feature f over the first n_sample rows of Y (f < d), in a
+// fixed summation order; part holds column_mean_blocks(n_sample) * d floats.
+void launch_column_mean(const float* Y, int64_t n_sample, int d, float* part, float* mu,
                         hipStream_t s);
+int64_t column_mean_blocks(int64_t n_sample);
 // dst[r] = bf16(src[r] - mu) zero-padded to dm; norm[r] = |dst[r]|^2 (fp32);
 // err[r] >= |src[r] - mu - dst[r]| (nullable).  With track_max the maxima of
 // norm and err go to sc (max_ynorm_key, max_yerr_key).

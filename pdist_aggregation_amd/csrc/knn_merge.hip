@@ -112,6 +112,23 @@ struct Vec4<double> {
 // found by ranking (each thread counts the entries ahead of its own, no
 // barriers); more than that are bitonic-sorted.
 constexpr int kRankMax = 128;
+// k' up to which the exact distances are ordered by ranking.
+constexpr int kRankExact = 256;
+// Split-list entries per thread held in registers (kMaxMerge / kThreads).
+constexpr int kPerThread = kMaxMerge / kThreads;
+static_assert(kPerThread * kThreads == kMaxMerge, "merge staging");
+
+// Stream compaction inside a wave: the slot of this lane among the lanes
+// with `pred`, after one atomicAdd per wave on the LDS counter.
+__device__ inline int wave_slot(bool pred, int* counter) {
+  const unsigned long long m = __ballot(pred);
+  const int lane = threadIdx.x & 63;
+  const int before = __builtin_popcountll(m & ((1ull << lane) - 1ull));
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(counter, __builtin_popcountll(m));
+  base = __shfl(base, 0, 64);
+  return base + before;
+}
 
 // One workgroup per query.
 template <class T>
@@ -127,7 +144,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   int* si = reinterpret_cast<int*>(dyn_smem + o_si);
   double* ed = reinterpret_cast<double*>(dyn_smem + o_ed);
   int* ei = reinterpret_cast<int*>(dyn_smem + o_ei);
-  __shared__ double xnorm_s, akc_s;
+  __shared__ double xnorm_s, akc_s, kth_s;
   __shared__ int n_s, full_s;
   __shared__ T cut_s;
 
@@ -138,51 +155,62 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   const int* pi = a.part_i + (size_t)q * total;
   // The query's final shared threshold bounds its k'-th value whenever the
   // search kept k' entries at or below it (a split reached k' entries there,
-  // or every entry passed the sampled guess): entries above it are dropped
-  // here.  Without k' entries at or below it the query fails the check below.
+  // or every entry passed the sampled guess): entries above it are dropped.
+  // Without k' entries at or below it the query fails the check below.
   const T tq = a.tau ? (T)from_ordered_key(a.tau[q]) : pos_inf<T>();
   // Each split's list is sorted, so the k'-th smallest of the first r
   // entries of every split (r * n_splits >= k') bounds the k'-th value from
   // above: entries past it are dropped too, which leaves a few more than k'.
-  const int r = a.kc < (a.kc + a.n_splits - 1) / a.n_splits + 1 ? a.kc
-                                                                 : (a.kc + a.n_splits - 1) / a.n_splits + 1;
-  if (tid == 0) {
-    n_s = 0;
-    cut_s = tq;
-  }
+  const int r0 = (a.kc + a.n_splits - 1) / a.n_splits + 1;
+  const int r = a.kc < r0 ? a.kc : r0;
+  T v[kPerThread];
+  int id[kPerThread];
+  if (tid == 0) n_s = 0;
   __syncthreads();
-  for (int e = tid; e < a.n_splits * r; e += kThreads) {
-    const int at = (e / r) * a.kc + e % r;
-    const int id = pi[at];
-    const T v = pd[at];
-    if (id >= 0 && id < a.n_t && v <= tq) {
-      const int h = atomicAdd(&n_s, 1);
-      sd[h] = v;
-      si[h] = id;
+#pragma unroll
+  for (int i = 0; i < kPerThread; ++i) {
+    const int e = tid + i * kThreads;
+    const bool in = e < total;
+    id[i] = in ? pi[e] : -1;
+    v[i] = in ? pd[e] : pos_inf<T>();
+    if (!(id[i] >= 0 && id[i] < a.n_t && v[i] <= tq)) id[i] = -1;  // never index past Y
+  }
+#pragma unroll
+  for (int i = 0; i < kPerThread; ++i) {
+    if (i * kThreads >= total) break;  // uniform
+    const int e = tid + i * kThreads;
+    const bool head = id[i] >= 0 && e % a.kc < r;
+    const int at = wave_slot(head, &n_s);
+    if (head) {
+      sd[at] = v[i];
+      si[at] = id[i];
     }
   }
   __syncthreads();
   const int nh = n_s;
+  if (tid == 0) cut_s = tq;
+  __syncthreads();
   if (nh >= a.kc) {
     for (int e = tid; e < nh; e += kThreads) {
-      const T v = sd[e];
-      const int id = si[e];
+      const T x = sd[e];
+      const int xi = si[e];
       int rank = 0;
-      for (int j = 0; j < nh; ++j) rank += key_less(sd[j], si[j], v, id) ? 1 : 0;
-      if (rank == a.kc - 1) cut_s = v;  // one thread: keys are distinct
+      for (int j = 0; j < nh; ++j) rank += key_less(sd[j], si[j], x, xi) ? 1 : 0;
+      if (rank == a.kc - 1) cut_s = x;  // one thread: keys are distinct
     }
   }
   __syncthreads();
   const T cut = cut_s;
   if (tid == 0) n_s = 0;
   __syncthreads();
-  for (int e = tid; e < total; e += kThreads) {
-    const int id = pi[e];
-    const T v = pd[e];
-    if (id >= 0 && id < a.n_t && v <= cut) {  // never index past Y
-      const int at = atomicAdd(&n_s, 1);
-      sd[at] = v;
-      si[at] = id;
+#pragma unroll
+  for (int i = 0; i < kPerThread; ++i) {
+    if (i * kThreads >= total) break;  // uniform
+    const bool keep = id[i] >= 0 && v[i] <= cut;
+    const int at = wave_slot(keep, &n_s);
+    if (keep) {
+      sd[at] = v[i];
+      si[at] = id[i];
     }
   }
   __syncthreads();
@@ -195,13 +223,13 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     // (value, index) keys are distinct (a train row is in one split only), so
     // ranks are a permutation.
     for (int e = tid; e < n; e += kThreads) {
-      const T v = sd[e];
-      const int id = si[e];
+      const T x = sd[e];
+      const int xi = si[e];
       int rank = 0;
-      for (int j = 0; j < n; ++j) rank += key_less(sd[j], si[j], v, id) ? 1 : 0;
+      for (int j = 0; j < n; ++j) rank += key_less(sd[j], si[j], x, xi) ? 1 : 0;
       if (rank < a.kc) {
-        od[rank] = v;
-        oi[rank] = id;
+        od[rank] = x;
+        oi[rank] = xi;
       }
     }
   } else {
@@ -252,10 +280,10 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   const int per_pass = kThreads / tpc;
   for (int j0 = 0; j0 < KP; j0 += per_pass) {
     const int j = j0 + tid / tpc, c = tid % tpc;
-    const int id = j < a.kc ? oi[j] : INT_MAX;
+    const int cid = j < a.kc ? oi[j] : INT_MAX;
     double s = 0.0;
-    if (id != INT_MAX) {
-      const T* yr = a.Y + (int64_t)id * a.d;
+    if (cid != INT_MAX) {
+      const T* yr = a.Y + (int64_t)cid * a.d;
       if (vec4) {  // 16-byte loads, every one of them in flight at once
         const int d4 = a.d >> 2;
 #pragma unroll 8
@@ -279,27 +307,54 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     for (int o = tpc >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     __syncthreads();  // every oi[] read of this pass precedes the ed/ei writes (shared storage)
     if (c == 0 && j < KP) {
-      ed[j] = id != INT_MAX ? s : __builtin_huge_val();
-      ei[j] = id;
+      ed[j] = cid != INT_MAX ? s : __builtin_huge_val();
+      ei[j] = cid;
     }
   }
   if (w == 0) {
     double s = 0.0;
     for (int p = lane; p < a.d; p += 64) {
-      const double v = (double)xr[p];
-      s = fma(v, v, s);
+      const double x = (double)xr[p];
+      s = fma(x, x, s);
     }
     s = wave_sum(s);
     if (lane == 0) xnorm_s = s;
   }
-  block_bitonic_sort(ed, ei, KP);
+  __syncthreads();
 
-  for (int j = tid; j < a.k; j += kThreads) {
-    const bool ok = j < a.kc && ei[j] != INT_MAX;
-    const double e = ok ? ed[j] : __builtin_huge_val();
-    a.idx_out[q * a.k + j] = ok ? (int64_t)ei[j] : -1;
-    a.dist_out[q * a.k + j] = (T)(a.squared ? e : sqrt(e));
+  // Order by (exact distance, index) and write the first k.  Up to
+  // kRankExact candidates: each one's rank is the number of candidates ahead
+  // of it (distinct keys); more: bitonic sort.
+  if (a.kc <= kRankExact) {
+    if (tid == 0) kth_s = __builtin_huge_val();
+    for (int j = tid; j < a.k; j += kThreads) {  // slots no candidate fills
+      a.idx_out[q * a.k + j] = -1;
+      a.dist_out[q * a.k + j] = pos_inf<T>();
+    }
+    __syncthreads();
+    for (int j = tid; j < a.kc; j += kThreads) {
+      if (ei[j] == INT_MAX) continue;
+      const double x = ed[j];
+      const int xi = ei[j];
+      int rank = 0;
+      for (int i = 0; i < a.kc; ++i) rank += key_less(ed[i], ei[i], x, xi) ? 1 : 0;
+      if (rank < a.k) {
+        a.idx_out[q * a.k + rank] = (int64_t)xi;
+        a.dist_out[q * a.k + rank] = (T)(a.squared ? x : sqrt(x));
+        if (rank == a.k - 1) kth_s = x;
+      }
+    }
+  } else {
+    block_bitonic_sort(ed, ei, KP);
+    for (int j = tid; j < a.k; j += kThreads) {
+      const bool ok = j < a.kc && ei[j] != INT_MAX;
+      const double x = ok ? ed[j] : __builtin_huge_val();
+      a.idx_out[q * a.k + j] = ok ? (int64_t)ei[j] : -1;
+      a.dist_out[q * a.k + j] = (T)(a.squared ? x : sqrt(x));
+    }
+    if (tid == 0) kth_s = a.k <= a.kc ? ed[a.k - 1] : __builtin_huge_val();
   }
+  __syncthreads();
 
   // Certification: every train row left out of the k' candidates has an
   // approximate value >= a_kc (the k'-th).  Its true distance is then at
@@ -311,14 +366,14 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
       const double akc = akc_s;
       const double nx = a.xnorm ? (double)a.xnorm[q] : xnorm_s;
       const double ny = from_ordered_key(a.sc->max_ynorm_key);
-      const double n = (double)a.acc_terms;
-      const double gamma = n * a.unit_roundoff / (1.0 - n * a.unit_roundoff);
+      const double nn = (double)a.acc_terms;
+      const double gamma = nn * a.unit_roundoff / (1.0 - nn * a.unit_roundoff);
       const double delta = gamma * (ny + 2.0 * sqrt(nx * ny) + fabs(akc) + nx);
       const double ex = a.xerr ? (double)a.xerr[q] : 0.0;
       const double ey = a.xerr ? from_ordered_key(a.sc->max_yerr_key) : 0.0;
       const double lo = akc + nx - delta;
       const double lower = sqrt(lo > 0.0 ? lo : 0.0) - ex - ey;
-      certified = lower > sqrt(ed[a.k - 1]);
+      certified = lower > sqrt(kth_s);
     }
     if (!certified) atomicAdd(&a.sc->n_uncertified, 1u);
   }

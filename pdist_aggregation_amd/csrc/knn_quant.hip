@@ -18,25 +18,38 @@ namespace {
 constexpr int kSampleRowsPerBlock = 256;
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+// Column means without floating-point atomics, so the centring vector --
+// and with it every quantised value -- is the same on every call: block b
+// writes the sums of its kSampleRowsPerBlock rows to part[b][f], then one
+// thread per feature adds the blocks in order.
 __global__ __launch_bounds__(kThreads) void column_sum_kernel(const float* __restrict__ Y,
                                                               int64_t n, int d,
-                                                              float* __restrict__ mu_sum) {
+                                                              float* __restrict__ part) {
   const int64_t r0 = (int64_t)blockIdx.x * kSampleRowsPerBlock;
   const int64_t r1 = r0 + kSampleRowsPerBlock < n ? r0 + kSampleRowsPerBlock : n;
   for (int f = threadIdx.x; f < d; f += kThreads) {
     float s = 0.f;
     for (int64_t r = r0; r < r1; ++r) s += Y[r * d + f];
-    atomicAdd(&mu_sum[f], s);
+    part[(size_t)blockIdx.x * d + f] = s;
   }
 }
 
-__global__ void column_scale_kernel(const float* __restrict__ mu_sum, int d, float inv,
-                                    float* __restrict__ mu) {
-  for (int f = threadIdx.x; f < d; f += blockDim.x) mu[f] = mu_sum[f] * inv;
-}
-
-__global__ void column_zero_kernel(float* __restrict__ p, int d) {
-  for (int f = threadIdx.x; f < d; f += blockDim.x) p[f] = 0.f;
+// One workgroup per feature: thread t adds blocks t, t + kThreads, ..., then a
+// fixed tree over the threads.
+__global__ __launch_bounds__(kThreads) void column_mean_kernel(const float* __restrict__ part,
+                                                               int n_blocks, int d, double inv,
+                                                               float* __restrict__ mu) {
+  __shared__ double red[kThreads];
+  const int f = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < n_blocks; b += kThreads) s += part[(size_t)b * d + f];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mu[f] = (float)(red[0] * inv);
 }
 
 // One wave per row.
@@ -209,8 +222,8 @@ constexpr int kVecRows = 4;
 
 __global__ __launch_bounds__(kThreads) void column_sum_vec_kernel(const float* __restrict__ Y,
                                                                   int64_t n, int d,
-                                                                  float* __restrict__ mu_sum) {
-  __shared__ f4 part[kThreads];
+                                                                  float* __restrict__ part) {
+  __shared__ f4 red[kThreads];
   const int groups = d >> 2;                 // float4 columns (<= 64)
   const int lanes_r = kThreads / groups;     // rows summed in parallel
   const int cg = threadIdx.x % groups, rl = threadIdx.x / groups;
@@ -225,20 +238,17 @@ __global__ __launch_bounds__(kThreads) void column_sum_vec_kernel(const float* _
       acc.z += v.z;
       acc.w += v.w;
     }
-  part[threadIdx.x] = acc;
+  red[threadIdx.x] = acc;
   __syncthreads();
   if (rl == 0) {
     for (int i = 1; i < lanes_r; ++i) {
-      const f4 v = part[i * groups + cg];
+      const f4 v = red[i * groups + cg];
       acc.x += v.x;
       acc.y += v.y;
       acc.z += v.z;
       acc.w += v.w;
     }
-    atomicAdd(&mu_sum[4 * cg + 0], acc.x);
-    atomicAdd(&mu_sum[4 * cg + 1], acc.y);
-    atomicAdd(&mu_sum[4 * cg + 2], acc.z);
-    atomicAdd(&mu_sum[4 * cg + 3], acc.w);
+    *reinterpret_cast<f4*>(part + (size_t)blockIdx.x * d + 4 * cg) = acc;
   }
 }
 
@@ -441,19 +451,20 @@ void launch_quantize_fp8(const float* src, int64_t n, int d, int dm, const float
                      track_max ? 1 : 0);
 }
 
-void launch_column_mean(const float* Y, int64_t n_sample, int d, float* mu_sum, float* mu,
+void launch_column_mean(const float* Y, int64_t n_sample, int d, float* part, float* mu,
                         hipStream_t s) {
-  hipLaunchKernelGGL(column_zero_kernel, dim3(1), dim3(kThreads), 0, s, mu_sum, d);
-  const int64_t blocks = ceil_div(n_sample, kSampleRowsPerBlock);
+  const int64_t blocks = column_mean_blocks(n_sample);
   if ((d & 3) == 0 && d <= 4 * 64 && (reinterpret_cast<uintptr_t>(Y) & 15) == 0)
     hipLaunchKernelGGL(column_sum_vec_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, Y,
-                       n_sample, d, mu_sum);
+                       n_sample, d, part);
   else
     hipLaunchKernelGGL(column_sum_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, Y,
-                       n_sample, d, mu_sum);
-  hipLaunchKernelGGL(column_scale_kernel, dim3(1), dim3(kThreads), 0, s, mu_sum, d,
-                     1.0f / (float)n_sample, mu);
+                       n_sample, d, part);
+  hipLaunchKernelGGL(column_mean_kernel, dim3((unsigned)d), dim3(kThreads), 0, s, part,
+                     (int)blocks, d, 1.0 / (double)n_sample, mu);
 }
+
+int64_t column_mean_blocks(int64_t n_sample) { return ceil_div(n_sample, kSampleRowsPerBlock); }
 
 void launch_quantize_bf16(const float* src, int64_t n, int d, int dm, const float* mu, void* dst,
                           float* norm, float* err, DeviceScalars* sc, bool track_max,

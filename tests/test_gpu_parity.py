@@ -336,3 +336,19 @@ def test_distance_squared_matrix(require_gpu, dtype, d):
     assert D.shape == (131, 517) and D.dtype == dtype
     ref = ((X.astype(np.float64)[:, None, :] - Y.astype(np.float64)[None]) ** 2).sum(-1)
     np.testing.assert_allclose(D, ref, rtol=1e-5 if dtype == np.float32 else 1e-13, atol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["exact", "bf16", "fp8"])
+def test_repeated_calls_identical(require_gpu, precision):
+    """Two calls on the same inputs return bit-identical rows (no order-
+    dependent floating-point reductions, e.g. in the centring vector)."""
+    rs = np.random.RandomState(17)
+    centres = rs.uniform(-1, 1, (200, 128))
+    Y = _gaussian_mixture(rs, 120000, 128, centres, 0.05)
+    Y[rs.choice(120000, 2000, replace=False)] = Y[rs.randint(0, 120000, 2000)]
+    X = _gaussian_mixture(rs, 2000, 128, centres, 0.05)
+    outs = [pa.parallel_knn(Y, X, 50, return_distance=True, squared=True, precision=precision,
+                            oversample=200)[0] for _ in range(3)]
+    for dist, idx in outs[1:]:
+        np.testing.assert_array_equal(idx, outs[0][1])
+        np.testing.assert_array_equal(dist, outs[0][0])
