@@ -35,19 +35,22 @@ __global__ void fill_u32_kernel(unsigned int* p, int64_t n, unsigned int v) {
     p[i] = v;
 }
 
-// One thread per query: the j-th smallest of the query's sampled minima
+// One wave per query: the j-th smallest of the query's sampled minima
 // (n_runs ascending runs of kSampleJ values), used as a guessed threshold.
+// Lane l keeps the kSampleJ smallest of runs l, l+64, ...; then j rounds of
+// "take the smallest head among the lanes" pick the j-th smallest overall.
 __global__ __launch_bounds__(kThreads) void sample_guess_kernel(const float* __restrict__ so,
                                                                 int64_t n_q, int n_runs, int j,
                                                                 unsigned long long* tau,
                                                                 float* guess, int fail_stride) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n_q) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t q = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (q >= n_q) return;  // whole wave
   float b[kSampleJ];
 #pragma unroll
   for (int i = 0; i < kSampleJ; ++i) b[i] = __builtin_huge_valf();
   const float* p = so + (size_t)q * n_runs * kSampleJ;
-  for (int r = 0; r < n_runs; ++r) {
+  for (int r = lane; r < n_runs; r += 64) {
     for (int i = 0; i < kSampleJ; ++i) {
       const float m = p[r * kSampleJ + i];
       if (!(m < b[kSampleJ - 1])) break;  // the rest of an ascending run is no better
@@ -56,15 +59,35 @@ __global__ __launch_bounds__(kThreads) void sample_guess_kernel(const float* __r
       b[0] = fminf(b[0], m);
     }
   }
+  // j rounds: the smallest head across the wave (ties: lowest lane) is taken
+  // and its lane shifts its list.
   float g = __builtin_huge_valf();
+  for (int round = 0; round < j; ++round) {
+    float m = b[0];
+    int who = lane;
 #pragma unroll
-  for (int i = 0; i < kSampleJ; ++i)
-    if (i == j - 1) g = b[i];
+    for (int o = 32; o > 0; o >>= 1) {
+      const float om = __shfl_xor(m, o, 64);
+      const int ow = __shfl_xor(who, o, 64);
+      if (om < m || (om == m && ow < who)) {
+        m = om;
+        who = ow;
+      }
+    }
+    g = m;
+    if (lane == who) {
+#pragma unroll
+      for (int t = 0; t < kSampleJ - 1; ++t) b[t] = b[t + 1];
+      b[kSampleJ - 1] = __builtin_huge_valf();
+    }
+  }
   if (fail_stride > 0 && q % fail_stride == 0) g = -__builtin_huge_valf();
-  guess[q] = g;
-  if (g < __builtin_huge_valf()) {
-    const unsigned long long key = ordered_key((double)g);
-    if (key < tau[q]) tau[q] = key;
+  if (lane == 0) {
+    guess[q] = g;
+    if (g < __builtin_huge_valf()) {
+      const unsigned long long key = ordered_key((double)g);
+      if (key < tau[q]) tau[q] = key;
+    }
   }
 }
 
@@ -405,8 +428,8 @@ void launch_fill_u32(unsigned int* p, int64_t n, unsigned int v, hipStream_t s) 
 void launch_sample_guess(const float* sample_out, int64_t n_q, int n_runs, int j,
                          unsigned long long* tau, float* guess, int fail_stride, hipStream_t s) {
   if (n_q <= 0) return;
-  hipLaunchKernelGGL(sample_guess_kernel, dim3((unsigned)ceil_div(n_q, kThreads)), dim3(kThreads),
-                     0, s, sample_out, n_q, n_runs, j, tau, guess, fail_stride);
+  hipLaunchKernelGGL(sample_guess_kernel, dim3((unsigned)ceil_div(n_q, kThreads / 64)),
+                     dim3(kThreads), 0, s, sample_out, n_q, n_runs, j, tau, guess, fail_stride);
 }
 
 size_t merge_lds_bytes(int n_splits, int kc, size_t elem) {
