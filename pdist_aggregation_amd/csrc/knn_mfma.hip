@@ -66,12 +66,42 @@ struct Fmt<kFmtFP8> {
   using breg = i32x8;
 };
 
-// LDS row pitch in bytes: DM elements + 16 bytes, so the 16-byte reads of 8
-// (or 16) consecutive rows start in distinct banks.
-template <int FMT, int DM>
-constexpr int row_pitch() {
-  return DM * Fmt<FMT>::esize + 16;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Direct-to-LDS buffer loads (lane l's bytes land at lds_addr + size * l),
+// written out so that the compiler does not order its own LDS reads behind
+// them: the sweep waits for them itself (vmcnt, then a barrier).  M0 takes
+// the LDS address, one wait state before the load.
+__device__ inline unsigned lds_address(const void* p) {
+  return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
+__device__ inline void lds_load16(unsigned lds_addr, int voff, i32x4 rsrc) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, 0 offen lds"
+               ::"s"(lds_addr), "v"(voff), "s"(rsrc) : "m0");
+}
+__device__ inline void lds_load4(unsigned lds_addr, int voff, i32x4 rsrc) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dword %1, %2, 0 offen lds"
+               ::"s"(lds_addr), "v"(voff), "s"(rsrc) : "m0");
+}
+// Buffer resource (raw, stride 0, range-checked against `bytes`).
+__device__ inline i32x4 make_rsrc(const void* base, int bytes) {
+  const uint64_t b = (uint64_t)(uintptr_t)base;
+  return i32x4{__builtin_amdgcn_readfirstlane((int)(unsigned)b),
+               __builtin_amdgcn_readfirstlane((int)((b >> 32) & 0xffff)),
+               __builtin_amdgcn_readfirstlane(bytes), 0x00020000};
+}
+
+// Tiles reach LDS by direct-to-LDS buffer loads (each lane's 16 bytes land at
+// the wave's LDS base + 16 * lane), three buffers deep.  Rows are not padded;
+// instead 16-byte chunk j of row r sits at chunk position j ^ (r & swz), so
+// the A-operand reads of 8 consecutive rows hit 8 different bank groups.
+template <int FMT, int DM>
+struct TileGeom {
+  static constexpr int ROWB = DM * Fmt<FMT>::esize;  // bytes per train row
+  static constexpr int NCH = ROWB / 16;               // 16-byte chunks per row
+  static constexpr int SWZ = NCH < 8 ? NCH - 1 : 7;
+};
+constexpr int kNBuf = 3;
 
 // MODE kModeSearch: the candidate search.  MODE kModeResearch: the same for
 // the query blocks / queries selected by qblock_active / query_active (a
@@ -86,21 +116,27 @@ constexpr int kModeResearch = 2;
 template <int FMT, int DM, int QG, int MODE>
 __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_kernel(MfmaArgs a) {
   using F = Fmt<FMT>;
-  constexpr int PITCH = row_pitch<FMT, DM>();
+  using G = TileGeom<FMT, DM>;
   constexpr int WQ = QG * 32;           // queries per wave
   constexpr int BQ = 4 * WQ;            // queries per workgroup
   constexpr int NSTEP = DM / (2 * F::KL);  // MFMAs per 32x32 tile
   constexpr int CHUNKS = kBY * DM * F::esize / 16;      // 16-byte loads per tile
   constexpr int CPT = (CHUNKS + kThreads - 1) / kThreads;  // per thread
 
-  __shared__ __align__(16) unsigned char ytile[2][kBY * PITCH];
-  __shared__ __align__(16) float yn_s[2][kBY];
-  __shared__ int ye_s[2][kBY];  // FP8: E8M0 row scales
+  // Three tile buffers as distinct objects: the compiler can tell the loads
+  // into one from the reads of another, and waits only where this code does.
+  struct alignas(16) TileBuf {
+    unsigned char y[kBY * G::ROWB];
+    float n[64];                                  // |y|^2 of the tile's rows (32 used)
+    unsigned char e[FMT == kFmtFP8 ? 256 : 16];   // FP8: E8M0 row scales (32 used)
+  };
+  __shared__ TileBuf tbuf0, tbuf1, tbuf2;
   __shared__ int cnt_s[2 * BQ];
   __shared__ float tau_s[BQ];
   extern __shared__ __align__(16) unsigned char dyn_smem[];  // 4 waves x cap x 8 B
 
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = lane >> 5, c = lane & 31;
+  const int tid = threadIdx.x, lane = tid & 63, h = lane >> 5, c = lane & 31;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index (uniform)
 
   // Workgroup -> (query block, split).  Workgroups are dealt to the 8 XCDs
   // round-robin; work items are numbered split-major and each XCD takes a
@@ -195,79 +231,70 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
   }
   for (int i = tid; i < BQ; i += kThreads) tau_s[i] = FLT_MAX;
 
-  // ---- tile loader: global -> registers -> LDS -------------------------------
-  // Two register sets: the tile two steps ahead is in flight while the next
-  // one is written to LDS.
-  struct Staged {
-    i32x4 v[CPT];
-    float n;
-    int e;
-  };
+  // ---- tile loader: global -> LDS (direct) ---------------------------------
   // Raw buffer loads over this split's rows: rows past the split read as
-  // zero (range-checked by the hardware), so the loads carry no branches and
-  // the compiler can wait for exactly the loads it needs.  The tile base is
-  // added to the per-lane offsets (the range check covers the vector offset;
-  // the scalar offset is kept at 0).
-  // Splits hold whole tiles except the last one of the train set, whose rows
-  // past n_t take |y|^2 = +inf from the padding after the norms (kNormPad
-  // entries): their values are +inf, never selected, with no mask here.  The
-  // norms of look-ahead tiles past a split are loaded but never used.
+  // zero (range-checked by the hardware), so no load needs a branch.  Splits
+  // hold whole tiles except the last one of the train set, whose rows past
+  // n_t take |y|^2 = +inf from the padding after the norms (kNormPad
+  // entries): their values are +inf, never selected.  The look-ahead tiles
+  // past a split are loaded but never used.
   const int row_bytes = a.ldy * F::esize;
   const int64_t nrows = yend - ybeg;
-  const __amdgpu_buffer_rsrc_t yrs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<unsigned char*>(reinterpret_cast<const unsigned char*>(a.Y)) +
-          ybeg * (int64_t)row_bytes,
-      (short)0, (int)(nrows * row_bytes), 0x00020000);
-  const __amdgpu_buffer_rsrc_t nrs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float*>(a.ynorm + ybeg), (short)0, (int)((nrows + 3 * kBY) * 4), 0x00020000);
-  __amdgpu_buffer_rsrc_t ers = nrs;
-  if constexpr (FMT == kFmtFP8)
-    ers = __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned char*>(a.yscale + ybeg), (short)0,
-                                            (int)nrows, 0x00020000);
-  int ld_off[CPT];
+  const i32x4 yrs = make_rsrc(reinterpret_cast<const unsigned char*>(a.Y) +
+                                  ybeg * (int64_t)row_bytes,
+                              (int)(nrows * row_bytes));
+  const i32x4 nrs = make_rsrc(a.ynorm + ybeg, (int)((nrows + 3 * kBY) * 4));
+  i32x4 ers = nrs;
+  if constexpr (FMT == kFmtFP8) ers = make_rsrc(a.yscale + ybeg, (int)nrows);
+  // Load instruction i of wave w fills LDS chunks [(4i + w) * 64, + 64); lane
+  // l brings the chunk for position (4i + w) * 64 + l.  Float32 features past
+  // d come from an offset past the buffer, i.e. zero.
+  constexpr int kOOB = 0x7ffffff0;
+  int voff[CPT];
 #pragma unroll
   for (int i = 0; i < CPT; ++i) {
-    const int ch = tid + i * kThreads;
-    ld_off[i] = (ch / (DM * F::esize / 16)) * row_bytes + (ch % (DM * F::esize / 16)) * 16;
+    const int slot = (i * 4 + w) * 64 + lane;
+    const int r = slot / G::NCH, j = (slot % G::NCH) ^ (r & G::SWZ);
+    voff[i] = (FMT == kFmtF32 && j * 4 >= a.d) ? -1 : r * row_bytes + j * 16;
   }
-  const int rl = tid & (kBY - 1);
-  auto load_tile = [&](Staged& st, int t) {  // tile t of this split
+  auto load_tile = [&](int t, TileBuf& b) {  // tile t of this split -> LDS buffer b
     const int tb = t * kBY;
 #pragma unroll
-    for (int i = 0; i < CPT; ++i)
-      st.v[i] = __builtin_amdgcn_raw_buffer_load_b128(yrs, ld_off[i] + tb * row_bytes, 0, 0);
-    st.n = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(nrs, (tb + rl) * 4, 0, 0));
-    st.e = 127;
-    if constexpr (FMT == kFmtFP8) st.e = (int)__builtin_amdgcn_raw_buffer_load_b8(ers, tb + rl, 0, 0);
-  };
-  // Masking happens here, not at the load, so nothing waits on the data
-  // before it is needed: float32 features past d read as 0.
-  auto store_tile = [&](const Staged& st, int buf) {
-#pragma unroll
     for (int i = 0; i < CPT; ++i) {
-      const int ch = tid + i * kThreads;
-      const int row = ch / (DM * F::esize / 16);
-      const int col = ch % (DM * F::esize / 16);
-      const int f = col * (16 / F::esize);
-      // BF16/FP8 rows are zero-padded to DM by the quantiser; only float32
-      // rows of d < DM features need the mask.
-      const i32x4 v = (FMT != kFmtF32 || f < a.d) ? st.v[i] : i32x4{0, 0, 0, 0};
-      if (CHUNKS % kThreads == 0 || ch < CHUNKS)
-        *reinterpret_cast<i32x4*>(&ytile[buf][row * PITCH + col * 16]) = v;
+      if ((i * 4 + w) * 64 >= CHUNKS) break;  // wave-uniform
+      const int vo = voff[i] < 0 ? kOOB : voff[i] + tb * row_bytes;
+      lds_load16(lds_address(&b.y[(i * 4 + w) * 64 * 16]), vo, yrs);
     }
-    // Every thread holds the norm of row (tid % 32); the 8 copies write the
-    // same value, without a branch.
-    yn_s[buf][rl] = st.n;
-    if constexpr (FMT == kFmtFP8) ye_s[buf][rl] = st.e;
+    if (w == 0)  // 64 norms: the tile's 32 rows and 32 unused
+      lds_load4(lds_address(&b.n[0]), (tb + lane) * 4, nrs);
+    if constexpr (FMT == kFmtFP8)
+      if (w == 1)  // 4 scale bytes per lane: the tile's 32 in lanes 0..7
+        lds_load4(lds_address(&b.e[0]), tb + 4 * lane, ers);
   };
+  // Loads each wave issues per tile, counting only what every wave issues:
+  // waiting until at most that many are outstanding completes every load of
+  // the tile before (in-order completion), then the barrier makes all waves'
+  // loads of that tile visible and frees the buffer read in the previous step.
+  // Wave w issues the load instructions of chunk groups w, w+4, ... of the
+  // CHUNKS/64 groups: the fewest, CHUNKS/256, are wave 3's.
+  constexpr int kMinLoads = CHUNKS / 256;
+  static_assert(CHUNKS % 64 == 0 && kMinLoads < 16, "tile load accounting");
+  // A-operand chunk addresses of this lane (row c), fixed for the sweep.
+  constexpr int NFRAG = FMT == kFmtF32 ? NSTEP / 4 : FMT == kFmtBF16 ? NSTEP : 2 * NSTEP;
+  int aoff[NFRAG];
+#pragma unroll
+  for (int f = 0; f < NFRAG; ++f) {
+    const int j = h * (G::NCH / 2) + f;
+    aoff[f] = (c * G::NCH + (j ^ (c & G::SWZ))) * 16;
+  }
 
   // ---- one 32-row tile on the matrix cores ------------------------------------
   // acc = |y|^2 (C operand of the first MFMA) - 2 x.y
-  auto compute = [&](f32x16(&acc)[QG], int buf) {
+  auto compute = [&](f32x16(&acc)[QG], const TileBuf& b) {
     f32x16 yn;
 #pragma unroll
     for (int r4 = 0; r4 < 4; ++r4) {
-      const f32x4 v = *reinterpret_cast<const f32x4*>(&yn_s[buf][r4 * 8 + h * 4]);
+      const f32x4 v = *reinterpret_cast<const f32x4*>(&b.n[r4 * 8 + h * 4]);
       yn[r4 * 4 + 0] = v[0];
       yn[r4 * 4 + 1] = v[1];
       yn[r4 * 4 + 2] = v[2];
@@ -275,11 +302,11 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
     }
 #pragma unroll
     for (int g = 0; g < QG; ++g) acc[g] = yn;
-    const unsigned char* arow = &ytile[buf][c * PITCH + h * (DM / 2) * F::esize];
+    const unsigned char* tile = &b.y[0];
     if constexpr (FMT == kFmtF32) {
 #pragma unroll
       for (int s = 0; s < NSTEP; s += 4) {
-        const f32x4 av = *reinterpret_cast<const f32x4*>(arow + s * 4);
+        const f32x4 av = *reinterpret_cast<const f32x4*>(tile + aoff[s / 4]);
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
@@ -289,7 +316,7 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
     } else if constexpr (FMT == kFmtBF16) {
 #pragma unroll
       for (int s = 0; s < NSTEP; ++s) {
-        const bf16x8 av = *reinterpret_cast<const bf16x8*>(arow + s * 16);
+        const bf16x8 av = *reinterpret_cast<const bf16x8*>(tile + aoff[s]);
 #pragma unroll
         for (int g = 0; g < QG; ++g)
           acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, xr[g][s], acc[g], 0, 0, 0);
@@ -297,11 +324,11 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
     } else {
       // Row scales ride in the MFMA (E8M0 per 32-element block): A's block is
       // one train row, B's one query, so D is already |y|^2 - 2 x.y.
-      const int sa = ye_s[buf][c];
+      const int sa = b.e[c];
 #pragma unroll
       for (int s = 0; s < NSTEP; ++s) {
-        const i32x4 lo = *reinterpret_cast<const i32x4*>(arow + s * 32);
-        const i32x4 hi = *reinterpret_cast<const i32x4*>(arow + s * 32 + 16);
+        const i32x4 lo = *reinterpret_cast<const i32x4*>(tile + aoff[2 * s]);
+        const i32x4 hi = *reinterpret_cast<const i32x4*>(tile + aoff[2 * s + 1]);
         const i32x8 av = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
 #pragma unroll
         for (int g = 0; g < QG; ++g)
@@ -499,26 +526,24 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
   // MFMA results (measured: 3.15 -> 2.48 ms at 1e4 x 1e6 x 128).
   // `sel(acc, y0, step)` consumes one finished tile.
   auto sweep = [&](auto&& sel) {
-    Staged stA, stB;
     f32x16 acc[QG];
-    load_tile(stA, 0);
-    store_tile(stA, 0);
-    load_tile(stB, 1);
-    __syncthreads();
-    // One step; `cur` is the LDS buffer of tile t, `ld` the free register
-    // set, `st` the set holding tile t+1.
-    auto one_step = [&](int t, Staged& ld, Staged& st) {
-      const int cur = t & 1;
-      load_tile(ld, t + 2);
+    load_tile(0, tbuf0);
+    load_tile(1, tbuf1);
+    auto step = [&](int t, const TileBuf& cur, TileBuf& ahead) {
+      // Tile t (loaded two steps ago) complete and visible; `ahead`, read in
+      // step t-1, free for tile t+2.
+      asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kMinLoads) : "memory");
+      load_tile(t + 2, ahead);
       compute(acc, cur);
-      store_tile(st, cur ^ 1);
       sel(acc, ybeg + (int64_t)t * kBY, t);
-      __syncthreads();
     };
-    for (int t = 0; t < ntiles; t += 2) {
-      one_step(t, stA, stB);
-      if (t + 1 < ntiles) one_step(t + 1, stB, stA);
+    for (int t = 0; t < ntiles; t += kNBuf) {
+      step(t, tbuf0, tbuf2);
+      if (t + 1 < ntiles) step(t + 1, tbuf1, tbuf0);
+      if (t + 2 < ntiles) step(t + 2, tbuf2, tbuf1);
     }
+    // No load may still be writing LDS when the workgroup ends.
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   };
 
   if constexpr (MODE == kModeSample) {
