@@ -10,8 +10,9 @@
 //     each lane compares its 16 values against ONE threshold;
 //   * the queries of a wave stay in registers for the whole sweep, already
 //     multiplied by -2 (exact), as the B operand;
-//   * train rows stream global -> registers -> LDS (double buffered, one
-//     barrier per 32-row step) and are read back in the A-operand layout;
+//   * train rows stream global -> LDS by direct-to-LDS buffer loads (three
+//     buffers, one wait + barrier per 32-row step) and are read back in the
+//     A-operand layout;
 //   * |y|^2 enters as the C operand of the first MFMA of each tile, so the
 //     accumulator IS the reduced distance and selection costs a min-tree of
 //     16 values and one compare per lane in the common case;
@@ -441,23 +442,33 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
                            fmin_ieee(acc[g][4 * r4 + 2], acc[g][4 * r4 + 3]));
       const float m = fmin_ieee(fmin_ieee(m4[0], m4[1]), fmin_ieee(m4[2], m4[3]));
       if (__any(!(m > tau[g]))) {
+        // Wave-wide votes (independent compares into scalar masks) pick the
+        // quads, then the values, that hold a survivor in any lane; a value
+        // with one is stored by every lane, the lanes without one at an
+        // offset past the buffer, which the range check drops (no divergent
+        // branch).
         const int ql = wq0 + g * 32 + c;
         const int base = (ql * a.cap + h * half) * 4;
         const int yb = (int)y0 + h * 4;
         int n = cnt[g];
+        uint64_t qm[4];
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) qm[r4] = __builtin_amdgcn_ballot_w64(!(m4[r4] > tau[g]));
 #pragma unroll
         for (int r4 = 0; r4 < 4; ++r4) {
-          if (!(m4[r4] > tau[g])) {
+          if (qm[r4] == 0) continue;
+          uint64_t vm[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const float v = acc[g][4 * r4 + e];
-              if (v <= tau[g]) {
-                const int off = base + n * 4;
-                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), cdrs, off, 0, 0);
-                __builtin_amdgcn_raw_buffer_store_b32(yb + r4 * 8 + e, cirs, off, 0, 0);
-                ++n;
-              }
-            }
+          for (int e = 0; e < 4; ++e) vm[e] = __builtin_amdgcn_ballot_w64(acc[g][4 * r4 + e] <= tau[g]);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if (vm[e] == 0) continue;
+            const float v = acc[g][4 * r4 + e];
+            const bool hit = v <= tau[g];
+            const int off = hit ? base + n * 4 : kOOB;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), cdrs, off, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(yb + r4 * 8 + e, cirs, off, 0, 0);
+            n += hit;
           }
         }
         cnt[g] = n;
@@ -508,17 +519,15 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
   };
 
   // ---- main sweep --------------------------------------------------------
-  // Step t (tile t in LDS buffer t%2, tile t+1 in flight in registers):
-  //   issue the loads of tile t+2; MFMAs of tile t; write tile t+1 to LDS;
-  //   select tile t; barrier.
+  // Step t (tile t in LDS buffer t%3, tile t+1 in flight):
+  //   wait for tile t + barrier; issue the loads of tile t+2 into the buffer
+  //   read in step t-1; MFMAs of tile t; select tile t.
   // The loads are issued unconditionally (tiles past the split read as zero
   // through the buffer range check), so every step issues the same number of
-  // them and writing tile t+1 waits only for tile t+1's own loads, never for
-  // the tile t+2 loads issued at the top of the same step.  Offsets stay
-  // below 2^31: a split holds at most 1 GiB of rows, plus the 95 rows of
-  // look-ahead.
+  // them and the wait for tile t leaves kMinLoads later loads in flight.
+  // Offsets stay below 2^31: a split holds at most 1 GiB of rows, plus the 95 rows
+  // of look-ahead.
   const int ntiles = yend > ybeg ? (int)((yend - ybeg + kBY - 1) / kBY) : 0;
-  // `sel(acc, y0, step)` consumes one finished tile.
   // One accumulator set: each tile is selected right after its own MFMAs.
   // Keeping a second set to select tile t-1 during tile t's MFMAs costs
   // 16*QG registers, which is the difference between 2 and 3 resident waves
