@@ -304,8 +304,11 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   for (int j0 = 0; j0 < KP; j0 += per_pass) {
     const int j = j0 + tid / tpc, c = tid % tpc;
     const int cid = j < a.kc ? oi[j] : INT_MAX;
+    // Every slot below min(n, k') holds a staged row when keys are distinct;
+    // the range test keeps the gather inside Y whatever the slot holds.
+    const bool valid = cid >= 0 && cid < a.n_t;
     double s = 0.0;
-    if (cid != INT_MAX) {
+    if (valid) {
       const T* yr = a.Y + (int64_t)cid * a.d;
       if (vec4) {  // 16-byte loads, every one of them in flight at once
         const int d4 = a.d >> 2;
@@ -330,8 +333,8 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     for (int o = tpc >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     __syncthreads();  // every oi[] read of this pass precedes the ed/ei writes (shared storage)
     if (c == 0 && j < KP) {
-      ed[j] = cid != INT_MAX ? s : __builtin_huge_val();
-      ei[j] = cid;
+      ed[j] = valid ? s : __builtin_huge_val();
+      ei[j] = valid ? cid : INT_MAX;
     }
   }
   if (w == 0) {
