@@ -76,13 +76,17 @@ typedef __attribute__((address_space(3))) void lds_void;
 __device__ inline unsigned lds_address(const void* p) {
   return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
-__device__ inline void lds_load16(unsigned lds_addr, int voff, i32x4 rsrc) {
-  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, 0 offen lds"
-               ::"s"(lds_addr), "v"(voff), "s"(rsrc) : "m0");
+// The LDS address is a scalar base plus an immediate, so the per-buffer
+// addresses need no registers of their own.
+template <int IMM>
+__device__ inline void lds_load16(unsigned lds_base, int voff, i32x4 rsrc) {
+  asm volatile("s_add_u32 m0, %0, %3\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, 0 offen lds"
+               ::"s"(lds_base), "v"(voff), "s"(rsrc), "i"(IMM) : "m0", "scc");
 }
-__device__ inline void lds_load4(unsigned lds_addr, int voff, i32x4 rsrc) {
-  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dword %1, %2, 0 offen lds"
-               ::"s"(lds_addr), "v"(voff), "s"(rsrc) : "m0");
+template <int IMM>
+__device__ inline void lds_load4(unsigned lds_base, int voff, i32x4 rsrc) {
+  asm volatile("s_add_u32 m0, %0, %3\n\ts_nop 0\n\tbuffer_load_dword %1, %2, 0 offen lds"
+               ::"s"(lds_base), "v"(voff), "s"(rsrc), "i"(IMM) : "m0", "scc");
 }
 // Buffer resource (raw, stride 0, range-checked against `bytes`).
 __device__ inline i32x4 make_rsrc(const void* base, int bytes) {
@@ -90,6 +94,14 @@ __device__ inline i32x4 make_rsrc(const void* base, int bytes) {
   return i32x4{__builtin_amdgcn_readfirstlane((int)(unsigned)b),
                __builtin_amdgcn_readfirstlane((int)((b >> 32) & 0xffff)),
                __builtin_amdgcn_readfirstlane(bytes), 0x00020000};
+}
+// The same resource moved `off` bytes on (scalar arithmetic only): the range
+// shrinks by as much, down to zero, so the hardware check still ends at the
+// original end.
+__device__ inline i32x4 rsrc_advance(i32x4 r, int off) {
+  const uint64_t b = ((uint64_t)(uint32_t)r[1] << 32 | (uint32_t)r[0]) + (uint32_t)off;
+  const int left = r[2] > off ? r[2] - off : 0;
+  return i32x4{(int)(uint32_t)b, (int)(uint32_t)(b >> 32), left, r[3]};
 }
 
 // Tiles reach LDS by direct-to-LDS buffer loads (each lane's 16 bytes land at
@@ -103,6 +115,30 @@ struct TileGeom {
   static constexpr int SWZ = NCH < 8 ? NCH - 1 : 7;
 };
 constexpr int kNBuf = 3;
+
+// fn(std::integral_constant<int, I>()) for I = 0 .. N-1.
+template <typename Fn, int... I>
+__device__ inline void static_for_impl(Fn& fn, std::integer_sequence<int, I...>) {
+  (fn(std::integral_constant<int, I>()), ...);
+}
+template <int N, typename Fn>
+__device__ inline void static_for(Fn&& fn) {
+  static_for_impl(fn, std::make_integer_sequence<int, N>());
+}
+
+// Wait until at most K vector-memory operations of this wave are
+// outstanding (lgkm: none), then the workgroup barrier.  vmcnt takes an
+// immediate, so a run-time count picks one of 16 forms; counts above B+15
+// wait for B+15, i.e. a little longer than needed.
+template <int K>
+__device__ inline void wait_barrier() {
+  asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(K) : "memory");
+}
+template <int B, int... I>
+__device__ inline void wait_barrier_upto(int n, std::integer_sequence<int, I...>) {
+  const int m = __builtin_amdgcn_readfirstlane(n < B ? B : (n > B + 15 ? B + 15 : n));
+  ((m == B + I ? wait_barrier<B + I>() : void()), ...);
+}
 
 // MODE kModeSearch: the candidate search.  MODE kModeResearch: the same for
 // the query blocks / queries selected by qblock_active / query_active (a
@@ -131,13 +167,14 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
     float n[64];                                  // |y|^2 of the tile's rows (32 used)
     unsigned char e[FMT == kFmtFP8 ? 256 : 16];   // FP8: E8M0 row scales (32 used)
   };
-  __shared__ TileBuf tbuf0, tbuf1, tbuf2;
+  __shared__ TileBuf tbuf[kNBuf];
   __shared__ int cnt_s[2 * BQ];
   __shared__ float tau_s[BQ];
   extern __shared__ __align__(16) unsigned char dyn_smem[];  // 4 waves x cap x 8 B
 
   const int tid = threadIdx.x, lane = tid & 63, h = lane >> 5, c = lane & 31;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index (uniform)
+  int n_stores = 0;  // candidate store instructions this wave issued (uniform)
 
   // Workgroup -> (query block, split).  Workgroups are dealt to the 8 XCDs
   // round-robin; work items are numbered split-major and each XCD takes a
@@ -248,29 +285,37 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
   i32x4 ers = nrs;
   if constexpr (FMT == kFmtFP8) ers = make_rsrc(a.yscale + ybeg, (int)nrows);
   // Load instruction i of wave w fills LDS chunks [(4i + w) * 64, + 64); lane
-  // l brings the chunk for position (4i + w) * 64 + l.  Float32 features past
-  // d come from an offset past the buffer, i.e. zero.
+  // l brings the chunk for position (4i + w) * 64 + l.  Each tile's loads go
+  // through resources moved to the tile (scalar work only), so a lane's
+  // offsets are the same for every tile.  Float32 features past d come from
+  // an offset past the buffer, i.e. zero.
   constexpr int kOOB = 0x7ffffff0;
   int voff[CPT];
 #pragma unroll
   for (int i = 0; i < CPT; ++i) {
     const int slot = (i * 4 + w) * 64 + lane;
     const int r = slot / G::NCH, j = (slot % G::NCH) ^ (r & G::SWZ);
-    voff[i] = (FMT == kFmtF32 && j * 4 >= a.d) ? -1 : r * row_bytes + j * 16;
+    voff[i] = (FMT == kFmtF32 && j * 4 >= a.d) ? kOOB : r * row_bytes + j * 16;
   }
-  auto load_tile = [&](int t, TileBuf& b) {  // tile t of this split -> LDS buffer b
+  const int noff = lane * 4, eoff = lane * 4;
+  // LDS byte address of this wave's first chunk in buffer 0.
+  const unsigned lds_w = lds_address(&tbuf[0].y[0]) + (unsigned)w * 64 * 16;
+  constexpr int kBufBytes = (int)sizeof(TileBuf);
+  constexpr int kNOff = (int)offsetof(TileBuf, n), kEOff = (int)offsetof(TileBuf, e);
+  auto load_tile = [&](int t, auto buf_c) {  // tile t of this split -> tbuf[buf_c]
+    constexpr int B = decltype(buf_c)::value;
     const int tb = t * kBY;
-#pragma unroll
-    for (int i = 0; i < CPT; ++i) {
-      if ((i * 4 + w) * 64 >= CHUNKS) break;  // wave-uniform
-      const int vo = voff[i] < 0 ? kOOB : voff[i] + tb * row_bytes;
-      lds_load16(lds_address(&b.y[(i * 4 + w) * 64 * 16]), vo, yrs);
-    }
+    const i32x4 yt = rsrc_advance(yrs, tb * row_bytes);
+    static_for<CPT>([&](auto ic) {
+      constexpr int I = decltype(ic)::value;
+      if ((I * 4 + w) * 64 >= CHUNKS) return;  // wave-uniform
+      lds_load16<B * kBufBytes + I * 4 * 64 * 16>(lds_w, voff[I], yt);
+    });
     if (w == 0)  // 64 norms: the tile's 32 rows and 32 unused
-      lds_load4(lds_address(&b.n[0]), (tb + lane) * 4, nrs);
+      lds_load4<B * kBufBytes + kNOff>(lds_w, noff, rsrc_advance(nrs, tb * 4));
     if constexpr (FMT == kFmtFP8)
       if (w == 1)  // 4 scale bytes per lane: the tile's 32 in lanes 0..7
-        lds_load4(lds_address(&b.e[0]), tb + 4 * lane, ers);
+        lds_load4<B * kBufBytes + kEOff - 1 * 64 * 16>(lds_w, eoff, rsrc_advance(ers, tb));
   };
   // Loads each wave issues per tile, counting only what every wave issues:
   // waiting until at most that many are outstanding completes every load of
@@ -469,6 +514,7 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), cdrs, off, 0, 0);
             __builtin_amdgcn_raw_buffer_store_b32(yb + r4 * 8 + e, cirs, off, 0, 0);
             n += hit;
+            n_stores += 2;
           }
         }
         cnt[g] = n;
@@ -534,22 +580,37 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
   // per SIMD at bf16 d=128; with 3 the other waves fill the wait for the
   // MFMA results (measured: 3.15 -> 2.48 ms at 1e4 x 1e6 x 128).
   // `sel(acc, y0, step)` consumes one finished tile.
+  //
+  // Stores count in vmcnt too, and complete in order with the loads: the
+  // candidate stores of steps t-2 and t-1 were issued after tile t's loads,
+  // so the wait for tile t leaves them (and tile t+1's loads) outstanding
+  // instead of waiting for stores issued moments ago.  Only stores counted
+  // in n_stores are allowed for; any other load or store issued since only
+  // makes the wait longer than needed, never too short.
   auto sweep = [&](auto&& sel) {
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    using B2 = std::integral_constant<int, 2>;
     f32x16 acc[QG];
-    load_tile(0, tbuf0);
-    load_tile(1, tbuf1);
-    auto step = [&](int t, const TileBuf& cur, TileBuf& ahead) {
+    load_tile(0, B0());
+    load_tile(1, B1());
+    // At step t's wait: n_stores when steps t-2 and t-1 began.
+    int st_m2 = 0, st_m1 = 0;
+    auto step = [&](int t, auto cur, auto ahead) {
       // Tile t (loaded two steps ago) complete and visible; `ahead`, read in
       // step t-1, free for tile t+2.
-      asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(kMinLoads) : "memory");
+      wait_barrier_upto<kMinLoads>(kMinLoads + n_stores - st_m2,
+                                   std::make_integer_sequence<int, 16>());
+      st_m2 = st_m1;
+      st_m1 = n_stores;
       load_tile(t + 2, ahead);
-      compute(acc, cur);
+      compute(acc, tbuf[decltype(cur)::value]);
       sel(acc, ybeg + (int64_t)t * kBY, t);
     };
     for (int t = 0; t < ntiles; t += kNBuf) {
-      step(t, tbuf0, tbuf2);
-      if (t + 1 < ntiles) step(t + 1, tbuf1, tbuf0);
-      if (t + 2 < ntiles) step(t + 2, tbuf2, tbuf1);
+      step(t, B0(), B2());
+      if (t + 1 < ntiles) step(t + 1, B1(), B0());
+      if (t + 2 < ntiles) step(t + 2, B2(), B1());
     }
     // No load may still be writing LDS when the workgroup ends.
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
