@@ -106,13 +106,19 @@ __device__ inline i32x4 rsrc_advance(i32x4 r, int off) {
 
 // Tiles reach LDS by direct-to-LDS buffer loads (each lane's 16 bytes land at
 // the wave's LDS base + 16 * lane), three buffers deep.  Rows are not padded;
-// instead 16-byte chunk j of row r sits at chunk position j ^ (r & swz), so
-// the A-operand reads of 8 consecutive rows hit 8 different bank groups.
+// instead 16-byte chunk j of row r sits at chunk position j ^ swz(r).  The
+// LDS has 64 banks of 4 bytes (256 bytes, 16 chunks), and a 16-byte read
+// serves 16 lanes at a time: the A-operand reads of rows r..r+15 (same
+// logical chunk) land on 16 different chunk columns when swz(r) is r's index
+// among the rows that share a column offset, i.e. r / R with R = 16 / NCH
+// rows per 256 bytes (R = 1 for rows of 256 bytes or more).
 template <int FMT, int DM>
 struct TileGeom {
   static constexpr int ROWB = DM * Fmt<FMT>::esize;  // bytes per train row
   static constexpr int NCH = ROWB / 16;               // 16-byte chunks per row
-  static constexpr int SWZ = NCH < 8 ? NCH - 1 : 7;
+  static constexpr int R = NCH < 16 ? 16 / NCH : 1;
+  static constexpr int MASK = (NCH < 16 ? NCH : 16) - 1;
+  __device__ static int swz(int r) { return (r / R) & MASK; }
 };
 constexpr int kNBuf = 3;
 
@@ -294,7 +300,7 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
 #pragma unroll
   for (int i = 0; i < CPT; ++i) {
     const int slot = (i * 4 + w) * 64 + lane;
-    const int r = slot / G::NCH, j = (slot % G::NCH) ^ (r & G::SWZ);
+    const int r = slot / G::NCH, j = (slot % G::NCH) ^ G::swz(r);
     voff[i] = (FMT == kFmtF32 && j * 4 >= a.d) ? kOOB : r * row_bytes + j * 16;
   }
   const int noff = lane * 4, eoff = lane * 4;
@@ -331,7 +337,7 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
 #pragma unroll
   for (int f = 0; f < NFRAG; ++f) {
     const int j = h * (G::NCH / 2) + f;
-    aoff[f] = (c * G::NCH + (j ^ (c & G::SWZ))) * 16;
+    aoff[f] = (c * G::NCH + (j ^ G::swz(c))) * 16;
   }
 
   // ---- one 32-row tile on the matrix cores ------------------------------------
