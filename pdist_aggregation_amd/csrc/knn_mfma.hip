@@ -366,12 +366,21 @@ __global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_
             acc[g] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], xr[g][s + e], acc[g], 0, 0, 0);
       }
     } else if constexpr (FMT == kFmtBF16) {
+      // Fragment reads three steps ahead of their MFMAs, kept there by
+      // scheduling barriers (left alone, the scheduler moves each read next
+      // to its first use and the MFMAs wait on LDS latency).
+      constexpr int PF = 3 < NSTEP ? 3 : NSTEP;
+      bf16x8 af[NSTEP];
+#pragma unroll
+      for (int s = 0; s < PF; ++s) af[s] = *reinterpret_cast<const bf16x8*>(tile + aoff[s]);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int s = 0; s < NSTEP; ++s) {
-        const bf16x8 av = *reinterpret_cast<const bf16x8*>(tile + aoff[s]);
+        if (s + PF < NSTEP) af[s + PF] = *reinterpret_cast<const bf16x8*>(tile + aoff[s + PF]);
 #pragma unroll
         for (int g = 0; g < QG; ++g)
-          acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, xr[g][s], acc[g], 0, 0, 0);
+          acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[s], xr[g][s], acc[g], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
       }
     } else {
       // Row scales ride in the MFMA (E8M0 per 32-element block): A's block is
