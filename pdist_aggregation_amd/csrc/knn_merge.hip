@@ -141,16 +141,47 @@ constexpr int kRankExact = 256;
 constexpr int kPerThread = kMaxMerge / kThreads;
 static_assert(kPerThread * kThreads == kMaxMerge, "merge staging");
 
-// Stream compaction inside a wave: the slot of this lane among the lanes
-// with `pred`, after one atomicAdd per wave on the LDS counter.
-__device__ inline int wave_slot(bool pred, int* counter) {
-  const unsigned long long m = __ballot(pred);
+// Stream compaction inside a wave over kPerThread rounds: at[i] = the slot
+// of this lane's round-i entry among all entries with pred[i] of the wave,
+// after ONE atomicAdd per wave on the LDS counter (the votes of every round
+// are taken first).  Slots only place entries; every later step orders by
+// key, so the order of waves and rounds does not matter.
+template <int N>
+__device__ inline void wave_slots(const bool (&pred)[N], int rounds, int* counter, int (&at)[N]) {
   const int lane = threadIdx.x & 63;
-  const int before = __builtin_popcountll(m & ((1ull << lane) - 1ull));
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long m[N];
+  int total = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    m[i] = i < rounds ? __ballot(pred[i]) : 0ull;
+    total += __builtin_popcountll(m[i]);
+  }
   int base = 0;
-  if (lane == 0 && m) base = atomicAdd(counter, __builtin_popcountll(m));
+  if (lane == 0 && total) base = atomicAdd(counter, total);
   base = __shfl(base, 0, 64);
-  return base + before;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    at[i] = base + __builtin_popcountll(m[i] & below);
+    base += __builtin_popcountll(m[i]);
+  }
+}
+
+// Number of keys (kd[j], ki[j]), j < n, ordered before (x, xi): four keys per
+// LDS read while four remain (kd, ki 16-byte aligned), then one at a time.
+template <class T>
+__device__ inline int count_less(const T* kd, const int* ki, int n, T x, int xi) {
+  using V4 = typename Vec4<T>::type;
+  int r = 0, j = 0;
+#pragma unroll 4
+  for (; j + 4 <= n; j += 4) {
+    const V4 d = *reinterpret_cast<const V4*>(kd + j);
+    const int4 i4 = *reinterpret_cast<const int4*>(ki + j);
+    r += (key_less(d.x, i4.x, x, xi) ? 1 : 0) + (key_less(d.y, i4.y, x, xi) ? 1 : 0) +
+         (key_less(d.z, i4.z, x, xi) ? 1 : 0) + (key_less(d.w, i4.w, x, xi) ? 1 : 0);
+  }
+  for (; j < n; ++j) r += key_less(kd[j], ki[j], x, xi) ? 1 : 0;
+  return r;
 }
 
 // One workgroup per query.
@@ -198,16 +229,22 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     v[i] = in ? pd[e] : pos_inf<T>();
     if (!(id[i] >= 0 && id[i] < a.n_t && v[i] <= tq)) id[i] = -1;  // never index past Y
   }
+  const int rounds = (total + kThreads - 1) / kThreads;  // uniform
+  {
+    bool head[kPerThread];
+    int at[kPerThread];
 #pragma unroll
-  for (int i = 0; i < kPerThread; ++i) {
-    if (i * kThreads >= total) break;  // uniform
-    const int e = tid + i * kThreads;
-    const bool head = id[i] >= 0 && e % a.kc < r;
-    const int at = wave_slot(head, &n_s);
-    if (head) {
-      sd[at] = v[i];
-      si[at] = id[i];
+    for (int i = 0; i < kPerThread; ++i) {
+      const int e = tid + i * kThreads;
+      head[i] = id[i] >= 0 && e % a.kc < r;
     }
+    wave_slots(head, rounds, &n_s, at);
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i)
+      if (head[i]) {
+        sd[at[i]] = v[i];
+        si[at[i]] = id[i];
+      }
   }
   __syncthreads();
   const int nh = n_s;
@@ -217,8 +254,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     for (int e = tid; e < nh; e += kThreads) {
       const T x = sd[e];
       const int xi = si[e];
-      int rank = 0;
-      for (int j = 0; j < nh; ++j) rank += key_less(sd[j], si[j], x, xi) ? 1 : 0;
+      const int rank = count_less(sd, si, nh, x, xi);
       if (rank == a.kc - 1) cut_s = x;  // one thread: keys are distinct
     }
   }
@@ -226,15 +262,18 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   const T cut = cut_s;
   if (tid == 0) n_s = 0;
   __syncthreads();
+  {
+    bool keep[kPerThread];
+    int at[kPerThread];
 #pragma unroll
-  for (int i = 0; i < kPerThread; ++i) {
-    if (i * kThreads >= total) break;  // uniform
-    const bool keep = id[i] >= 0 && v[i] <= cut;
-    const int at = wave_slot(keep, &n_s);
-    if (keep) {
-      sd[at] = v[i];
-      si[at] = id[i];
-    }
+    for (int i = 0; i < kPerThread; ++i) keep[i] = id[i] >= 0 && v[i] <= cut;
+    wave_slots(keep, rounds, &n_s, at);
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i)
+      if (keep[i]) {
+        sd[at[i]] = v[i];
+        si[at[i]] = id[i];
+      }
   }
   __syncthreads();
   const int n = n_s;
@@ -248,8 +287,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     for (int e = tid; e < n; e += kThreads) {
       const T x = sd[e];
       const int xi = si[e];
-      int rank = 0;
-      for (int j = 0; j < n; ++j) rank += key_less(sd[j], si[j], x, xi) ? 1 : 0;
+      const int rank = count_less(sd, si, n, x, xi);
       if (rank < a.kc) {
         od[rank] = x;
         oi[rank] = xi;
@@ -362,8 +400,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
       if (ei[j] == INT_MAX) continue;
       const double x = ed[j];
       const int xi = ei[j];
-      int rank = 0;
-      for (int i = 0; i < a.kc; ++i) rank += key_less(ed[i], ei[i], x, xi) ? 1 : 0;
+      const int rank = count_less(ed, ei, a.kc, x, xi);
       if (rank < a.k) {
         a.idx_out[q * a.k + rank] = (int64_t)xi;
         a.dist_out[q * a.k + rank] = (T)(a.squared ? x : sqrt(x));
