@@ -137,9 +137,13 @@ struct Vec4<double> {
 constexpr int kRankMax = 128;
 // k' up to which the exact distances are ordered by ranking.
 constexpr int kRankExact = 256;
-// Split-list entries per thread held in registers (kMaxMerge / kThreads).
-constexpr int kPerThread = kMaxMerge / kThreads;
-static_assert(kPerThread * kThreads == kMaxMerge, "merge staging");
+// Merge workgroup: two waves per query, so twice as many queries are in
+// flight per CU as with four (each phase holds at most a few entries per
+// thread, so the smaller group does not lengthen a query).
+constexpr int kMergeThreads = 128;
+// Split-list entries per thread held in registers (kMaxMerge / kMergeThreads).
+constexpr int kPerThread = kMaxMerge / kMergeThreads;
+static_assert(kPerThread * kMergeThreads == kMaxMerge, "merge staging");
 
 // Stream compaction inside a wave over kPerThread rounds: at[i] = the slot
 // of this lane's round-i entry among all entries with pred[i] of the wave,
@@ -186,7 +190,7 @@ __device__ inline int count_less(const T* kd, const int* ki, int n, T x, int xi)
 
 // One workgroup per query.
 template <class T>
-__global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) {
+__global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T> a) {
   extern __shared__ __align__(16) unsigned char dyn_smem[];
   const int total = a.n_splits * a.kc;
   const int P = next_pow2(total);
@@ -223,19 +227,19 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kPerThread; ++i) {
-    const int e = tid + i * kThreads;
+    const int e = tid + i * kMergeThreads;
     const bool in = e < total;
     id[i] = in ? pi[e] : -1;
     v[i] = in ? pd[e] : pos_inf<T>();
     if (!(id[i] >= 0 && id[i] < a.n_t && v[i] <= tq)) id[i] = -1;  // never index past Y
   }
-  const int rounds = (total + kThreads - 1) / kThreads;  // uniform
+  const int rounds = (total + kMergeThreads - 1) / kMergeThreads;  // uniform
   {
     bool head[kPerThread];
     int at[kPerThread];
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i) {
-      const int e = tid + i * kThreads;
+      const int e = tid + i * kMergeThreads;
       head[i] = id[i] >= 0 && e % a.kc < r;
     }
     wave_slots(head, rounds, &n_s, at);
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   if (tid == 0) cut_s = tq;
   __syncthreads();
   if (nh >= a.kc) {
-    for (int e = tid; e < nh; e += kThreads) {
+    for (int e = tid; e < nh; e += kMergeThreads) {
       const T x = sd[e];
       const int xi = si[e];
       const int rank = count_less(sd, si, nh, x, xi);
@@ -284,7 +288,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   if (n <= kRankMax) {
     // (value, index) keys are distinct (a train row is in one split only), so
     // ranks are a permutation.
-    for (int e = tid; e < n; e += kThreads) {
+    for (int e = tid; e < n; e += kMergeThreads) {
       const T x = sd[e];
       const int xi = si[e];
       const int rank = count_less(sd, si, n, x, xi);
@@ -295,13 +299,13 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     }
   } else {
     const int Pn = next_pow2(n);
-    for (int e = n + tid; e < Pn; e += kThreads) {
+    for (int e = n + tid; e < Pn; e += kMergeThreads) {
       sd[e] = pos_inf<T>();
       si[e] = INT_MAX;
     }
     block_bitonic_sort(sd, si, Pn);
   }
-  for (int j = n + tid; j < a.kc; j += kThreads) {
+  for (int j = n + tid; j < a.kc; j += kMergeThreads) {
     od[j] = pos_inf<T>();
     oi[j] = INT_MAX;
   }
@@ -333,12 +337,13 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   // every candidate row's loads are in flight together), then a shuffle
   // reduction over the tpc lanes.
   const T* xr = a.X + q * a.d;
-  const int tpc = KP >= kThreads ? 1 : (kThreads / KP < 64 ? kThreads / KP : 64);  // 2^i <= 64
+  const int tpc = KP >= kMergeThreads ? 1
+                  : (kMergeThreads / KP < 64 ? kMergeThreads / KP : 64);  // 2^i <= 64
   using V4 = typename Vec4<T>::type;
   const bool vec4 = (a.d & 3) == 0 &&
                     ((reinterpret_cast<uintptr_t>(a.X) | reinterpret_cast<uintptr_t>(a.Y)) &
                      (4 * sizeof(T) - 1)) == 0;
-  const int per_pass = kThreads / tpc;
+  const int per_pass = kMergeThreads / tpc;
   for (int j0 = 0; j0 < KP; j0 += per_pass) {
     const int j = j0 + tid / tpc, c = tid % tpc;
     const int cid = j < a.kc ? oi[j] : INT_MAX;
@@ -391,12 +396,12 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
   // of it (distinct keys); more: bitonic sort.
   if (a.kc <= kRankExact) {
     if (tid == 0) kth_s = __builtin_huge_val();
-    for (int j = tid; j < a.k; j += kThreads) {  // slots no candidate fills
+    for (int j = tid; j < a.k; j += kMergeThreads) {  // slots no candidate fills
       a.idx_out[q * a.k + j] = -1;
       a.dist_out[q * a.k + j] = pos_inf<T>();
     }
     __syncthreads();
-    for (int j = tid; j < a.kc; j += kThreads) {
+    for (int j = tid; j < a.kc; j += kMergeThreads) {
       if (ei[j] == INT_MAX) continue;
       const double x = ed[j];
       const int xi = ei[j];
@@ -409,7 +414,7 @@ __global__ __launch_bounds__(kThreads) void merge_rerank_kernel(MergeArgs<T> a) 
     }
   } else {
     block_bitonic_sort(ed, ei, KP);
-    for (int j = tid; j < a.k; j += kThreads) {
+    for (int j = tid; j < a.k; j += kMergeThreads) {
       const bool ok = j < a.kc && ei[j] != INT_MAX;
       const double x = ok ? ed[j] : __builtin_huge_val();
       a.idx_out[q * a.k + j] = ok ? (int64_t)ei[j] : -1;
@@ -482,7 +487,7 @@ template <class T>
 void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s) {
   if (a.n_q <= 0) return;
   const size_t lds = merge_lds_bytes(a.n_splits, a.kc, sizeof(T));
-  hipLaunchKernelGGL(merge_rerank_kernel<T>, dim3((unsigned)a.n_q), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL(merge_rerank_kernel<T>, dim3((unsigned)a.n_q), dim3(kMergeThreads), lds, s, a);
 }
 
 template void launch_fill_keys<float>(unsigned long long*, int64_t, unsigned long long, hipStream_t);
