@@ -535,8 +535,6 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     if (p.guess) {
       auto* failed = reinterpret_cast<int*>(ws + p.off_failed);
       auto* failed_qb = reinterpret_cast<int*>(ws + p.off_failed_qb);
-      knn::launch_fill_u32(reinterpret_cast<unsigned*>(failed), n_test, 0u, stream);
-      knn::launch_fill_u32(reinterpret_cast<unsigned*>(failed_qb), p.n_qblocks, 0u, stream);
       knn::MfmaArgs sa = fa;
       sa.n_splits = p.sample_regions;
       sa.rows_per_split = p.sample_rows;
@@ -547,12 +545,12 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
         if (owned) (void)hipFreeAsync(ws, stream);
         return fail(KNN_EUNSUPPORTED, "no MFMA kernel for this feature count");
       }
-      knn::launch_sample_guess(sa.sample_out, n_test, 2 * p.sample_regions, p.sample_j, tau,
-                               reinterpret_cast<float*>(ws + p.off_guess), p.guess_fail_stride,
-                               stream);
       ma.failed = failed;
       ma.failed_qblocks = failed_qb;
       ma.qblock_queries = knn::mfma_block_queries(fa.fmt, p.dm);
+      knn::launch_sample_guess(sa.sample_out, n_test, 2 * p.sample_regions, p.sample_j, tau,
+                               reinterpret_cast<float*>(ws + p.off_guess), p.guess_fail_stride,
+                               failed, failed_qb, ma.qblock_queries, stream);
     }
     // 3. search
     timer.mark(1);

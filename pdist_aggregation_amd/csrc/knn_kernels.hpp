@@ -114,9 +114,12 @@ bool launch_mfma_sample(const MfmaArgs& a, hipStream_t s);
 // guess[q] = the j-th smallest of the sampled minima of query q (ascending
 // runs of kSampleJ, n_runs per query); tau[q] <- min(tau[q], key(guess[q])).
 // fail_stride > 0 forces guess -inf on queries q % fail_stride == 0, which
-// then fail the check and are searched again (exercises that path).
+// then fail the check and are searched again (exercises that path).  Also
+// clears the failure flags the merge sets: failed[q] and, from the first
+// query of each block of qblock_queries, failed_qblocks[q / qblock_queries].
 void launch_sample_guess(const float* sample_out, int64_t n_q, int n_runs, int j,
-                         unsigned long long* tau, float* guess, int fail_stride, hipStream_t s);
+                         unsigned long long* tau, float* guess, int fail_stride, int* failed,
+                         int* failed_qblocks, int qblock_queries, hipStream_t s);
 void launch_fill_u32(unsigned int* p, int64_t n, unsigned int v, hipStream_t s);
 // Queries per workgroup of the (fmt, dm) instantiation (4 waves x 32 x groups).
 int mfma_block_queries(int fmt, int dm);

@@ -39,20 +39,36 @@ __global__ void fill_u32_kernel(unsigned int* p, int64_t n, unsigned int v) {
 // (n_runs ascending runs of kSampleJ values), used as a guessed threshold.
 // Lane l keeps the kSampleJ smallest of runs l, l+64, ...; then j rounds of
 // "take the smallest head among the lanes" pick the j-th smallest overall.
-__global__ __launch_bounds__(kThreads) void sample_guess_kernel(const float* __restrict__ so,
-                                                                int64_t n_q, int n_runs, int j,
-                                                                unsigned long long* tau,
-                                                                float* guess, int fail_stride) {
+__global__ __launch_bounds__(kThreads) void sample_guess_kernel(
+    const float* __restrict__ so, int64_t n_q, int n_runs, int j, unsigned long long* tau,
+    float* guess, int fail_stride, int* failed, int* failed_qblocks, int qblock_queries) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  static_assert(kSampleJ % 4 == 0, "runs are read as float4");
   const int lane = threadIdx.x & 63;
   const int64_t q = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
   if (q >= n_q) return;  // whole wave
+  if (lane == 0 && failed) {
+    failed[q] = 0;
+    if (q % qblock_queries == 0) failed_qblocks[q / qblock_queries] = 0;
+  }
   float b[kSampleJ];
 #pragma unroll
   for (int i = 0; i < kSampleJ; ++i) b[i] = __builtin_huge_valf();
   const float* p = so + (size_t)q * n_runs * kSampleJ;
   for (int r = lane; r < n_runs; r += 64) {
+    // The whole run in four 16-byte loads, then merged from registers.
+    float run[kSampleJ];
+#pragma unroll
+    for (int i = 0; i < kSampleJ; i += 4) {
+      const f4 v = *reinterpret_cast<const f4*>(p + (size_t)r * kSampleJ + i);
+      run[i] = v[0];
+      run[i + 1] = v[1];
+      run[i + 2] = v[2];
+      run[i + 3] = v[3];
+    }
+#pragma unroll
     for (int i = 0; i < kSampleJ; ++i) {
-      const float m = p[r * kSampleJ + i];
+      const float m = run[i];
       if (!(m < b[kSampleJ - 1])) break;  // the rest of an ascending run is no better
 #pragma unroll
       for (int t = kSampleJ - 1; t > 0; --t) b[t] = fminf(b[t], fmaxf(b[t - 1], m));
@@ -471,10 +487,12 @@ void launch_fill_u32(unsigned int* p, int64_t n, unsigned int v, hipStream_t s) 
 }
 
 void launch_sample_guess(const float* sample_out, int64_t n_q, int n_runs, int j,
-                         unsigned long long* tau, float* guess, int fail_stride, hipStream_t s) {
+                         unsigned long long* tau, float* guess, int fail_stride, int* failed,
+                         int* failed_qblocks, int qblock_queries, hipStream_t s) {
   if (n_q <= 0) return;
   hipLaunchKernelGGL(sample_guess_kernel, dim3((unsigned)ceil_div(n_q, kThreads / 64)),
-                     dim3(kThreads), 0, s, sample_out, n_q, n_runs, j, tau, guess, fail_stride);
+                     dim3(kThreads), 0, s, sample_out, n_q, n_runs, j, tau, guess, fail_stride,
+                     failed, failed_qblocks, qblock_queries);
 }
 
 size_t merge_lds_bytes(int n_splits, int kc, size_t elem) {
