@@ -26,6 +26,32 @@ PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "fp64": 78.6}  # TFLOP/s, 
 HBM_PEAK = 8000.0  # GB/s
 
 
+def measured_ceiling(head):
+    """Sustained MFMA rate of this GPU for the head's instruction (bf16
+    32x32x16 or fp32 32x32x2), from libpdist_probe.so: independent MFMA chains
+    from registers only, 1..4 workgroups of 4 waves per CU, best of the four.
+    Reported beside the datasheet peak, which assumes the maximum clock."""
+    fmt = {"bf16": 0, "fp32": 1}.get(head)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pdist_aggregation_amd",
+                        "libpdist_probe.so")
+    if fmt is None or not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.pdist_probe_mfma_tflops.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)] * 2
+    best = None
+    for per_cu in (1, 2, 3, 4):
+        tf, ms = ctypes.c_double(), ctypes.c_double()
+        rc = lib.pdist_probe_mfma_tflops(fmt, per_cu, 1000 if fmt == 0 else 500, 5, ctypes.byref(tf),
+                                         ctypes.byref(ms))
+        if rc != 0:
+            return None
+        if best is None or tf.value > best[0]:
+            best = (tf.value, per_cu, ms.value)
+    return {"value": best[0], "unit": "TFLOP/s", "workgroups_per_cu": best[1],
+            "ms_per_launch": best[2],
+            "how": "v_mfma chains from registers, no memory traffic (libpdist_probe.so)"}
+
+
 def make_inputs(nq, nt, d, seed=0):
     g = np.random.default_rng(seed)
     Y = g.random((nt, d), dtype=np.float32)
@@ -280,6 +306,10 @@ def main():
     }
     if "fp32" in ok and "bf16" in ok:
         line["bf16_speedup_vs_fp32"] = ok["bf16"]["queries_per_s"] / ok["fp32"]["queries_per_s"]
+    ceiling = measured_ceiling(head)
+    if ceiling:
+        line["roofline"]["measured_peak"] = ceiling
+        line["roofline"]["frac_of_measured_peak"] = achieved / ceiling["value"]
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(Y, X, args.k, args.cpu_seconds)

@@ -1,4 +1,6 @@
-"""Build libpdist_knn.so (the HIP kernels and the C ABI) for gfx950, in-tree.
+"""Build libpdist_knn.so (the HIP kernels and the C ABI) for gfx950, in-tree,
+and libpdist_probe.so (the matrix-core rate probe bench.py reports beside the
+datasheet peak; not part of the C ABI).
 
 Usage: ``python -m pdist_aggregation_amd.build`` (or call :func:`build`).
 hipcc cross-compiles for gfx950 without a GPU present.
@@ -14,6 +16,8 @@ import sys
 PKG = pathlib.Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpdist_knn.so"
+PROBE = PKG / "probe" / "mfma_ceiling.hip"
+PROBE_LIB = PKG / "libpdist_probe.so"
 ARCH = os.environ.get("PDIST_KNN_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -39,11 +43,10 @@ def hipcc():
     raise RuntimeError("hipcc not found (ROCm is required to build libpdist_knn.so)")
 
 
-def build(verbose: bool = False) -> pathlib.Path:
-    """Compile every csrc/*.hip into one shared library next to this file."""
-    tmp = LIB.with_suffix(".so.tmp")
+def _compile(srcs, out: pathlib.Path, verbose: bool) -> None:
+    tmp = out.with_suffix(".so.tmp")
     cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(CSRC), "-I", str(PKG.parent / "include"),
-           *map(str, sources()), "-o", str(tmp)]
+           *map(str, srcs), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -53,7 +56,14 @@ def build(verbose: bool = False) -> pathlib.Path:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
     if verbose and err.strip():
         print(err, file=sys.stderr)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> pathlib.Path:
+    """Compile every csrc/*.hip into one shared library next to this file, and
+    the rate probe into its own."""
+    _compile(sources(), LIB, verbose)
+    _compile([PROBE], PROBE_LIB, verbose)
     return LIB
 
 

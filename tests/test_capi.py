@@ -108,3 +108,18 @@ def test_effective_n_threads_env(monkeypatch):
     assert pa.effective_n_threads() == 3
     monkeypatch.delenv("OMP_NUM_THREADS")
     assert 1 <= pa.effective_n_threads() <= (__import__("os").cpu_count() or 1)
+
+
+def test_rate_probe_library_loads_and_rejects_bad_arguments():
+    """libpdist_probe.so (the matrix-core rate bench.py reports beside the
+    datasheet peak) loads without a GPU; argument checks fail before any HIP
+    call."""
+    path = pathlib.Path(_capi.LIB_PATH).with_name("libpdist_probe.so")
+    lib = ctypes.CDLL(str(path))
+    fn = lib.pdist_probe_mfma_tflops
+    fn.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_double)] * 2
+    tf, ms = ctypes.c_double(), ctypes.c_double()
+    assert fn(2, 1, 10, 1, ctypes.byref(tf), ctypes.byref(ms)) == -1  # unknown format
+    assert fn(0, 0, 10, 1, ctypes.byref(tf), ctypes.byref(ms)) == -1  # per_cu out of range
+    assert fn(0, 5, 10, 1, ctypes.byref(tf), ctypes.byref(ms)) == -1
+    assert fn(0, 1, 0, 1, ctypes.byref(tf), ctypes.byref(ms)) == -1  # no iterations
