@@ -149,15 +149,19 @@ double binom_tail(int n, double f, int j) {
 // Sampled-threshold plan.  The k'-1 rows ranked above a query's k'-th value
 // each fall in a sample of fraction f with probability f (rows in random
 // order), so the j-th smallest sampled value lies below the k'-th value --
-// the guess is too tight -- with probability P(Bin(k'-1, f) >= j) (minima
-// of 16 rows only raise the guess).  j is the smallest with that bound at
-// most kGuessFail; a query whose guess fails is searched again, so the bound
-// only governs speed.  PDIST_KNN_GUESS: "0" off, "1" on at any size,
+// the guess is too tight -- with probability P(Bin(k'-1, f) >= j).  Two
+// reductions only raise the guess, never lower it: the sample keeps minima
+// of 16 rows, and each sampling lane keeps only its kSampleJ smallest
+// minima (so j may exceed kSampleJ: the j-th smallest of the kept values is
+// at least the j-th smallest of all).  j is the smallest with that bound at
+// most kGuessFail, up to kGuessMaxJ rounds of the guess kernel; a query whose
+// guess fails is searched again, so the bound only governs speed.  PDIST_KNN_GUESS: "0" off, "1" on at any size,
 // "fail:S" on with the guess of every S-th query forced to fail (exercises
 // the re-search; "fail" = every query).
 constexpr int64_t kGuessMinRows = 131072;
 constexpr int kSampleShift = 5;  // sample 1/32 of the train rows
 constexpr double kGuessFail = 1e-7;
+constexpr int kGuessMaxJ = 128;
 
 void plan_guess(Plan& p, int64_t slots) {
   const char* env = std::getenv("PDIST_KNN_GUESS");
@@ -169,19 +173,28 @@ void plan_guess(Plan& p, int64_t slots) {
   if (mode == "0" || (!forced && p.n_train < kGuessMinRows)) return;
   const int64_t tiles = knn::ceil_div(p.n_train, (int64_t)knn::kMfmaRowStep);
   const int64_t want = std::max<int64_t>(1, knn::ceil_div(tiles, (int64_t)1 << kSampleShift));
+  // Regions: enough sampling workgroups to fill the GPU, and at least j/kSampleJ
+  // of them so that the kept minima (2 halves x regions x kSampleJ per query)
+  // hold at least 2j values: the first pass finds j at the fraction of the
+  // GPU-filling layout, the second settles the layout.
   int64_t regions = std::max<int64_t>(1, slots / std::max<int64_t>(p.n_qblocks, 1));
-  regions = std::min<int64_t>({regions, want, 64});
-  const int64_t per = knn::ceil_div(want, regions);                 // tiles per region
-  const int64_t stride = std::max<int64_t>(per, tiles / regions);   // tiles between regions
-  p.sample_regions = (int)regions;
-  p.sample_rows = per * knn::kMfmaRowStep;
-  p.sample_stride = stride * knn::kMfmaRowStep;
-  p.sample_fraction = std::min(1.0, (double)(regions * per) / (double)tiles);
   int j = 1;
-  while (j <= knn::kSampleJ && binom_tail(p.kc - 1, p.sample_fraction, j) > kGuessFail) ++j;
-  if (j > knn::kSampleJ) {
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1) regions = std::max<int64_t>(regions, knn::ceil_div((int64_t)j, knn::kSampleJ));
+    regions = std::min<int64_t>({regions, want, 64});
+    const int64_t per = knn::ceil_div(want, regions);                // tiles per region
+    const int64_t stride = std::max<int64_t>(per, tiles / regions);  // tiles between regions
+    p.sample_regions = (int)regions;
+    p.sample_rows = per * knn::kMfmaRowStep;
+    p.sample_stride = stride * knn::kMfmaRowStep;
+    p.sample_fraction = std::min(1.0, (double)(regions * per) / (double)tiles);
+    j = 1;
+    while (j <= kGuessMaxJ && binom_tail(p.kc - 1, p.sample_fraction, j) > kGuessFail) ++j;
+  }
+  const int jmax = (int)std::min<int64_t>(kGuessMaxJ, 2 * regions * knn::kSampleJ);
+  if (j > jmax) {
     if (!forced) return;
-    j = knn::kSampleJ;
+    j = jmax;
   }
   p.sample_j = j;
   p.guess_fail_bound = binom_tail(p.kc - 1, p.sample_fraction, j);
