@@ -200,6 +200,15 @@ struct MergeArgs {
   unsigned long long* tau;
   // Re-search merge: only queries with only_queries[q] != 0 (nullptr: all).
   const int* only_queries;
+  // Pre-merge (pre_d != nullptr): workgroup (q, g) reduces the lists of
+  // splits [g * group_size, + group_size) of query q (n_splits in all) to its
+  // k' best in ascending (value, index) order, written to
+  // pre_d / pre_i[(q * n_groups + g) * kc + j] (padding +inf / -1); no exact
+  // re-rank, no failure check.  A second (final) merge then reads n_groups
+  // lists per query, which lifts the n_splits * k' <= kMaxMerge limit.
+  T* pre_d;
+  int* pre_i;
+  int group_size, n_groups;
 };
 
 template <class T>

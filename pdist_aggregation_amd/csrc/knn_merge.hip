@@ -204,12 +204,20 @@ __device__ inline int count_less(const T* kd, const int* ki, int n, T x, int xi)
   return r;
 }
 
-// One workgroup per query.
+// One workgroup per query (final merge) or per (query, group of splits)
+// (pre-merge, a.pre_d set).
 template <class T>
 __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T> a) {
   extern __shared__ __align__(16) unsigned char dyn_smem[];
-  const int total = a.n_splits * a.kc;
-  const int P = next_pow2(total);
+  const bool pre = a.pre_d != nullptr;
+  const int64_t q = pre ? (int64_t)blockIdx.x / a.n_groups : (int64_t)blockIdx.x;
+  const int grp = pre ? (int)(blockIdx.x % a.n_groups) : 0;
+  // Lists merged here, and the list count the LDS layout was sized for.
+  const int lists_alloc = pre ? a.group_size : a.n_splits;
+  const int first = grp * lists_alloc;
+  const int lists = pre ? min(a.group_size, a.n_splits - first) : a.n_splits;
+  const int total = lists * a.kc;
+  const int P = next_pow2(lists_alloc * a.kc);
   const int KP = next_pow2(a.kc);
   const size_t o_si = align16((size_t)P * sizeof(T));
   const size_t o_ed = align16(o_si + (size_t)P * sizeof(int));
@@ -222,11 +230,10 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
   __shared__ int n_s, full_s;
   __shared__ T cut_s;
 
-  const int64_t q = blockIdx.x;
   if (a.only_queries && a.only_queries[q] == 0) return;  // re-search merge: not selected
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const T* pd = a.part_d + (size_t)q * total;
-  const int* pi = a.part_i + (size_t)q * total;
+  const T* pd = a.part_d + ((size_t)q * a.n_splits + first) * a.kc;
+  const int* pi = a.part_i + ((size_t)q * a.n_splits + first) * a.kc;
   // The query's final shared threshold bounds its k'-th value whenever the
   // search kept k' entries at or below it (a split reached k' entries there,
   // or every entry passed the sampled guess): entries above it are dropped.
@@ -235,7 +242,7 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
   // Each split's list is sorted, so the k'-th smallest of the first r
   // entries of every split (r * n_splits >= k') bounds the k'-th value from
   // above: entries past it are dropped too, which leaves a few more than k'.
-  const int r0 = (a.kc + a.n_splits - 1) / a.n_splits + 1;
+  const int r0 = (a.kc + lists - 1) / lists + 1;
   const int r = a.kc < r0 ? a.kc : r0;
   T v[kPerThread];
   int id[kPerThread];
@@ -326,6 +333,14 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
     oi[j] = INT_MAX;
   }
   __syncthreads();
+  if (pre) {  // the group's k' best, sorted, for the final merge
+    const size_t o = ((size_t)q * a.n_groups + grp) * a.kc;
+    for (int j = tid; j < a.kc; j += kMergeThreads) {
+      a.pre_d[o + j] = od[j];
+      a.pre_i[o + j] = oi[j] == INT_MAX ? -1 : oi[j];
+    }
+    return;
+  }
 
   // Sampled-threshold check: with at least min(k', n_t) candidates every row
   // the guess dropped has a value above the k'-th kept one, so nothing is
@@ -504,8 +519,10 @@ size_t merge_lds_bytes(int n_splits, int kc, size_t elem) {
 template <class T>
 void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s) {
   if (a.n_q <= 0) return;
-  const size_t lds = merge_lds_bytes(a.n_splits, a.kc, sizeof(T));
-  hipLaunchKernelGGL(merge_rerank_kernel<T>, dim3((unsigned)a.n_q), dim3(kMergeThreads), lds, s, a);
+  const size_t lds = merge_lds_bytes(a.pre_d ? a.group_size : a.n_splits, a.kc, sizeof(T));
+  const int64_t blocks = a.pre_d ? a.n_q * a.n_groups : a.n_q;
+  hipLaunchKernelGGL(merge_rerank_kernel<T>, dim3((unsigned)blocks), dim3(kMergeThreads), lds, s,
+                     a);
 }
 
 template void launch_fill_keys<float>(unsigned long long*, int64_t, unsigned long long, hipStream_t);
