@@ -69,6 +69,10 @@ struct StageTimer {
 struct Plan {
   int64_t n_train = 0, n_test = 0, dim = 0, k = 0;
   int kc = 0, cap = 0, n_splits = 1, path = 0, dm = 0;
+  // Two-level merge: with more than kMaxMerge / k' splits, groups of
+  // group_size split lists are first reduced to n_groups lists per query.
+  int group_size = 1, n_groups = 1;
+  size_t off_pre_d = 0, off_pre_i = 0;
   int64_t rows_per_split = 0, n_qblocks = 0;
   size_t elem = 4;
   // Sampled threshold (MFMA paths): a first launch over sample_regions row
@@ -89,15 +93,20 @@ struct Plan {
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
               kPathLowd = 4;
 
+// Lists one merge workgroup takes (its register staging holds kMaxMerge
+// entries), and the most splits the two-level merge can take.
+int64_t merge_lists(int kc) { return std::max(1, knn::kMaxMerge / kc); }
+int64_t merge_split_limit(int kc) { return merge_lists(kc) * merge_lists(kc); }
+
 // Split count for a grid of n_qblocks query blocks: enough workgroups to keep
 // every CU busy several times over, each split at least min_steps row steps,
-// the merged per-query list within the merge kernel's LDS, and (MFMA) a
-// multiple of 8 so a split's query blocks share one XCD.
+// within what the two-level merge takes, and (MFMA) a multiple of 8 so a
+// split's query blocks share one XCD.
 int choose_splits(int64_t n_qblocks, int64_t n_train, int row_step, int kc, int64_t target,
                   bool multiple_of_8) {
   int64_t splits = std::max<int64_t>(1, (target + n_qblocks / 2) / n_qblocks);
   splits = std::min<int64_t>(splits, std::max<int64_t>(1, n_train / (8 * row_step)));
-  splits = std::min<int64_t>(splits, std::max(1, knn::kMaxMerge / kc));
+  splits = std::min<int64_t>(splits, merge_split_limit(kc));
   if (multiple_of_8 && splits >= 8) splits = splits / 8 * 8;
   return (int)std::max<int64_t>(splits, 1);
 }
@@ -120,9 +129,9 @@ int choose_splits_slots(int64_t n_qblocks, int64_t n_train, int row_step, int kc
                         int64_t slots) {
   if (const char* env = std::getenv("PDIST_KNN_SPLITS")) {
     const int v = std::atoi(env);
-    if (v > 0) return v;
+    if (v > 0) return (int)std::min<int64_t>(v, merge_split_limit(kc));
   }
-  int64_t smax = std::min<int64_t>(n_train / (8 * row_step), knn::kMaxMerge / kc);
+  int64_t smax = std::min<int64_t>(n_train / (8 * row_step), merge_split_limit(kc));
   smax = std::max<int64_t>(1, std::min<int64_t>(smax, 512));
   auto cost = [&](int64_t s) {
     return (double)knn::ceil_div(n_qblocks * s, std::max<int64_t>(slots, 1)) / (double)s;
@@ -155,10 +164,11 @@ double binom_tail(int n, double f, int j) {
 // minima (so j may exceed kSampleJ: the j-th smallest of the kept values is
 // at least the j-th smallest of all).  j is the smallest with that bound at
 // most kGuessFail, up to kGuessMaxJ rounds of the guess kernel; a query whose
-// guess fails is searched again, so the bound only governs speed.  PDIST_KNN_GUESS: "0" off, "1" on at any size,
-// "fail:S" on with the guess of every S-th query forced to fail (exercises
-// the re-search; "fail" = every query).
-constexpr int64_t kGuessMinRows = 131072;
+// guess fails is searched again, so the bound only governs speed.
+// PDIST_KNN_GUESS: "0" off, "1" on at any size, "fail:S" on with the guess of
+// every S-th query forced to fail (exercises the re-search; "fail" = every
+// query).
+constexpr int64_t kGuessMinRows = 32768;
 constexpr int kSampleShift = 5;  // sample 1/32 of the train rows
 constexpr double kGuessFail = 1e-7;
 constexpr int kGuessMaxJ = 128;
@@ -170,7 +180,11 @@ void plan_guess(Plan& p, int64_t slots) {
   if (mode == "fail") fail_stride = 1;
   if (mode.rfind("fail:", 0) == 0) fail_stride = std::max(1, std::atoi(mode.c_str() + 5));
   const bool forced = mode == "1" || fail_stride > 0;
-  if (mode == "0" || (!forced && p.n_train < kGuessMinRows)) return;
+  // Off below kGuessMinRows train rows (the sampling launch and the guess
+  // cost more than they save), unless a split holds fewer than 64 k' rows:
+  // started from +inf, its lists would fill and be compacted over and over.
+  const bool small_splits = p.rows_per_split < 64 * (int64_t)p.kc;
+  if (mode == "0" || (!forced && p.n_train < kGuessMinRows && !small_splits)) return;
   const int64_t tiles = knn::ceil_div(p.n_train, (int64_t)knn::kMfmaRowStep);
   const int64_t want = std::max<int64_t>(1, knn::ceil_div(tiles, (int64_t)1 << kSampleShift));
   // Regions: enough sampling workgroups to fill the GPU, and at least j/kSampleJ
@@ -301,6 +315,14 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   }
   p.rows_per_split = knn::ceil_div(knn::ceil_div(n_train, splits), step) * step;
   p.n_splits = (int)knn::ceil_div(n_train, p.rows_per_split);
+  if (p.n_splits > merge_split_limit(p.kc))
+    return fail(KNN_EUNSUPPORTED, "k' = " + std::to_string(p.kc) + " with " +
+                                      std::to_string(p.n_splits) +
+                                      " train splits exceeds the two-level merge");
+  if (p.n_splits > merge_lists(p.kc)) {
+    p.group_size = (int)knn::ceil_div((int64_t)p.n_splits, merge_lists(p.kc));
+    p.n_groups = (int)knn::ceil_div((int64_t)p.n_splits, (int64_t)p.group_size);
+  }
   const bool mfma = p.path == kPathMfmaF32 || p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8;
   if (mfma) plan_guess(p, slots);
 
@@ -321,6 +343,11 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   const size_t npart = (size_t)nq1 * p.n_splits * p.kc;
   p.off_part_d = take(npart * celem);
   p.off_part_i = take(npart * sizeof(int));
+  if (p.group_size > 1) {
+    const size_t npre = (size_t)nq1 * p.n_groups * p.kc;
+    p.off_pre_d = take(npre * celem);
+    p.off_pre_i = take(npre * sizeof(int));
+  }
   if (mfma && !p.guess) p.off_slots = take((size_t)nq1 * knn::align8(p.n_splits) * sizeof(float));
   if (p.guess) {
     p.off_sample = take((size_t)nq1 * p.sample_regions * 2 * knn::kSampleJ * sizeof(float));
@@ -424,6 +451,10 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
                                0ull, stream);
   knn::launch_fill_keys<float>(tau, n_test, knn::ordered_key(__builtin_huge_val()), stream);
   knn::MergeArgs<T> ma;
+  ma.pre_d = nullptr;
+  ma.pre_i = nullptr;
+  ma.group_size = 1;
+  ma.n_groups = 1;
   ma.failed = nullptr;
   ma.failed_qblocks = nullptr;
   ma.qblock_queries = 1;
@@ -588,7 +619,25 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.sc = sc;
   ma.dist_out = dist_out;
   ma.idx_out = idx_out;
-  knn::launch_merge_rerank<T>(ma, stream);
+  // With more splits than one merge workgroup takes, groups of split lists
+  // are first reduced to one list each (pre-merge), then merged.
+  auto merge = [&](const knn::MergeArgs<T>& m0) {
+    knn::MergeArgs<T> m = m0;
+    if (p.group_size > 1) {
+      knn::MergeArgs<T> pm = m0;
+      pm.pre_d = reinterpret_cast<T*>(ws + p.off_pre_d);
+      pm.pre_i = reinterpret_cast<int*>(ws + p.off_pre_i);
+      pm.group_size = p.group_size;
+      pm.n_groups = p.n_groups;
+      pm.failed = nullptr;
+      knn::launch_merge_rerank<T>(pm, stream);
+      m.part_d = pm.pre_d;
+      m.part_i = pm.pre_i;
+      m.n_splits = p.n_groups;
+    }
+    knn::launch_merge_rerank<T>(m, stream);
+  };
+  merge(ma);
   if constexpr (sizeof(T) == 4) {
     // 5. queries whose sampled guess proved too tight: search again (only
     // their query blocks, only them), from tau = +inf, and merge them.
@@ -600,7 +649,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       knn::MergeArgs<T> rm = ma;
       rm.failed = nullptr;
       rm.only_queries = ma.failed;
-      knn::launch_merge_rerank<T>(rm, stream);
+      merge(rm);
     }
   }
   timer.mark(3);
