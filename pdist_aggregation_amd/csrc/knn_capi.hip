@@ -173,7 +173,9 @@ constexpr int kSampleShift = 5;  // sample 1/32 of the train rows
 constexpr double kGuessFail = 1e-7;
 constexpr int kGuessMaxJ = 128;
 
-void plan_guess(Plan& p, int64_t slots) {
+// runs: sampled runs per (query, region) the kernel writes (2 for the MFMA
+// kernel's two lane halves, 1 for the low-d kernel's one lane per query).
+void plan_guess(Plan& p, int64_t slots, int runs) {
   const char* env = std::getenv("PDIST_KNN_GUESS");
   const std::string mode = env ? env : "";
   int fail_stride = 0;
@@ -187,14 +189,15 @@ void plan_guess(Plan& p, int64_t slots) {
   if (mode == "0" || (!forced && p.n_train < kGuessMinRows && !small_splits)) return;
   const int64_t tiles = knn::ceil_div(p.n_train, (int64_t)knn::kMfmaRowStep);
   const int64_t want = std::max<int64_t>(1, knn::ceil_div(tiles, (int64_t)1 << kSampleShift));
-  // Regions: enough sampling workgroups to fill the GPU, and at least j/kSampleJ
-  // of them so that the kept minima (2 halves x regions x kSampleJ per query)
-  // hold at least 2j values: the first pass finds j at the fraction of the
-  // GPU-filling layout, the second settles the layout.
+  // Regions: enough sampling workgroups to fill the GPU, and enough of them
+  // that the kept minima (runs x regions x kSampleJ per query) hold at least
+  // 2j values: the first pass finds j at the fraction of the GPU-filling
+  // layout, the second settles the layout.
   int64_t regions = std::max<int64_t>(1, slots / std::max<int64_t>(p.n_qblocks, 1));
   int j = 1;
   for (int pass = 0; pass < 2; ++pass) {
-    if (pass == 1) regions = std::max<int64_t>(regions, knn::ceil_div((int64_t)j, knn::kSampleJ));
+    if (pass == 1)
+      regions = std::max<int64_t>(regions, knn::ceil_div(2 * (int64_t)j, runs * knn::kSampleJ));
     regions = std::min<int64_t>({regions, want, 64});
     const int64_t per = knn::ceil_div(want, regions);                // tiles per region
     const int64_t stride = std::max<int64_t>(per, tiles / regions);  // tiles between regions
@@ -205,7 +208,7 @@ void plan_guess(Plan& p, int64_t slots) {
     j = 1;
     while (j <= kGuessMaxJ && binom_tail(p.kc - 1, p.sample_fraction, j) > kGuessFail) ++j;
   }
-  const int jmax = (int)std::min<int64_t>(kGuessMaxJ, 2 * regions * knn::kSampleJ);
+  const int jmax = (int)std::min<int64_t>(kGuessMaxJ, runs * regions * knn::kSampleJ);
   if (j > jmax) {
     if (!forced) return;
     j = jmax;
@@ -324,7 +327,8 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     p.n_groups = (int)knn::ceil_div((int64_t)p.n_splits, (int64_t)p.group_size);
   }
   const bool mfma = p.path == kPathMfmaF32 || p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8;
-  if (mfma) plan_guess(p, slots);
+  if (mfma) plan_guess(p, slots, 2);
+  if (p.path == kPathLowd && elem == 4) plan_guess(p, slots, 1);
 
   const size_t celem = (p.path == kPathGeneric || p.path == kPathLowd) ? elem : sizeof(float);
   const int64_t nq1 = std::max<int64_t>(n_test, 1);
@@ -465,10 +469,10 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.xerr = nullptr;
   ma.unit_roundoff = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
   ma.acc_terms = 2 * (int)dim + 4;
+  knn::SearchArgs<T> sa{};  // generic / low-d search (value-initialised: a plain search)
   if (p.path == kPathGeneric || p.path == kPathLowd) {
     // 2. norms  3. search
     knn::launch_row_norms<T>(X_train, n_train, (int)dim, ynorm, sc, stream);
-    knn::SearchArgs<T> sa;
     sa.X = X_test;
     sa.Y = X_train;
     sa.ynorm = ynorm;
@@ -485,6 +489,26 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     sa.tau = tau;
     sa.part_d = reinterpret_cast<T*>(ws + p.off_part_d);
     sa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
+    // 2b. sampled threshold (low-d, float): minima over 1/32 of the rows
+    // -> guess -> tau, as for the MFMA kernels (one run per query and region).
+    if constexpr (sizeof(T) == 4) {
+      if (p.guess) {
+        auto* failed = reinterpret_cast<int*>(ws + p.off_failed);
+        auto* failed_qb = reinterpret_cast<int*>(ws + p.off_failed_qb);
+        knn::SearchArgs<float> sm = sa;
+        sm.n_splits = p.sample_regions;
+        sm.rows_per_split = p.sample_rows;
+        sm.split_stride = p.sample_stride;
+        sm.sample_out = reinterpret_cast<float*>(ws + p.off_sample);
+        knn::launch_lowd_sample(sm, stream);
+        ma.failed = failed;
+        ma.failed_qblocks = failed_qb;
+        ma.qblock_queries = knn::kThreads;
+        knn::launch_sample_guess(sm.sample_out, n_test, p.sample_regions, p.sample_j, tau,
+                                 reinterpret_cast<float*>(ws + p.off_guess), p.guess_fail_stride,
+                                 failed, failed_qb, ma.qblock_queries, stream);
+      }
+    }
     timer.mark(1);
     if (p.path == kPathLowd)
       knn::launch_lowd_search<T>(sa, stream);
@@ -642,10 +666,17 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     // 5. queries whose sampled guess proved too tight: search again (only
     // their query blocks, only them), from tau = +inf, and merge them.
     if (ma.failed) {
-      knn::MfmaArgs ra = fa;
-      ra.qblock_active = ma.failed_qblocks;
-      ra.query_active = ma.failed;
-      knn::launch_mfma_research(ra, stream);
+      if (p.path == kPathLowd) {
+        knn::SearchArgs<float> ra = sa;
+        ra.qblock_active = ma.failed_qblocks;
+        ra.query_active = ma.failed;
+        knn::launch_lowd_research(ra, stream);
+      } else {
+        knn::MfmaArgs ra = fa;
+        ra.qblock_active = ma.failed_qblocks;
+        ra.query_active = ma.failed;
+        knn::launch_mfma_research(ra, stream);
+      }
       knn::MergeArgs<T> rm = ma;
       rm.failed = nullptr;
       rm.only_queries = ma.failed;

@@ -45,6 +45,15 @@ struct SearchArgs {
   unsigned long long* tau;  // [n_q] ordered key of the best known k'-th value
   T* part_d;                // [n_q][n_splits][kc] sorted candidate lists
   int* part_i;
+  // Low-d kernel only (zero / nullptr: a plain search).  split_stride: rows
+  // between the starts of consecutive splits (0: rows_per_split).
+  // sample_out: the sampling launch's per-query runs, [n_q][n_splits][kSampleJ]
+  // smallest 32-row minima, ascending.  qblock_active / query_active: the
+  // re-search launch's query blocks and queries.
+  int64_t split_stride;
+  float* sample_out;
+  const int* qblock_active;
+  const int* query_active;
 };
 
 // ---------------------------------------------------------------------------
@@ -163,6 +172,11 @@ template <class T>
 using LowdArgs = SearchArgs<T>;
 template <class T>
 bool launch_lowd_search(const LowdArgs<T>& a, hipStream_t s);
+// Sampling launch (rows [s * split_stride, + rows_per_split) of each of the
+// n_splits regions, writing sample_out) and re-search launch (qblock_active /
+// query_active) of the same kernel; float only.
+bool launch_lowd_sample(const LowdArgs<float>& a, hipStream_t s);
+bool launch_lowd_research(const LowdArgs<float>& a, hipStream_t s);
 template <class T>
 int lowd_blocks_per_cu(int d, int cap);
 

@@ -4,9 +4,10 @@
 // At d = 3 a GEMM formulation wastes the matrix cores (K would be padded
 // 3 -> 32) and the generic kernel's 16-feature LDS slices waste 13/16 of its
 // FMAs.  Here every lane owns ONE query whose -2 x sits in registers; the
-// workgroup stages 32 train rows per step in LDS and every lane walks all 32
-// rows (broadcast LDS reads), so a candidate costs d FMAs plus its share of
-// a min-tree.  Selection is the same contract as the MFMA kernel: append
+// workgroup stages 1024 train elements (256 rows at d <= 4) per stage in LDS,
+// double buffered, and every lane walks them 32 rows at a time (broadcast LDS
+// reads), so a candidate costs d FMAs plus its share of a min-tree, and each
+// stage's global loads have a whole stage of compute to arrive in.  Selection is the same contract as the MFMA kernel: append
 // v = |y|^2 - 2 x.y <= tau to the query's list in global scratch, compact
 // with a wave-local sort, share tau across splits with atomicMin.
 #include "knn_common.hpp"
@@ -16,12 +17,24 @@ namespace knn {
 
 namespace {
 
-constexpr int kRows = 32;  // train rows per step
+constexpr int kRows = 32;  // train rows per step (one pass of the value registers)
+// Train rows per LDS stage: 1024 elements, i.e. 4 per thread.
+template <int DP>
+constexpr int stage_rows() { return 1024 / DP; }
 
-template <class T, int DP>
+// MODE kLowdSearch: the candidate search.  kLowdResearch: the same for the
+// query blocks / queries selected by qblock_active / query_active.
+// kLowdSample: the same sweep over sampled row ranges, keeping each lane's
+// kSampleJ smallest 32-row minima (a threshold estimate, no lists).
+constexpr int kLowdSearch = 0, kLowdSample = 1, kLowdResearch = 2;
+
+template <class T, int DP, int MODE>
 __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
-  __shared__ T ys[2][kRows][DP];
-  __shared__ T yn_s[2][kRows];
+  constexpr int SR = stage_rows<DP>();
+  constexpr int EPT = SR * DP / kThreads;  // staged elements per thread
+  static_assert(SR * DP == EPT * kThreads && SR <= kThreads && SR % kRows == 0, "stage shape");
+  __shared__ T ys[2][SR][DP];
+  __shared__ T yn_s[2][SR];
   __shared__ int cnt_s[kThreads];
   extern __shared__ __align__(16) unsigned char dyn_smem[];  // 4 waves x cap x (T + int)
 
@@ -35,10 +48,11 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
     split = w_item / a.n_qblocks;
     qb = w_item % a.n_qblocks;
   }
+  if (MODE == kLowdResearch && a.qblock_active[qb] == 0) return;  // block not selected
   const int64_t q0 = (int64_t)qb * kThreads;
   const int64_t q = q0 + tid;
   const bool qok = q < a.n_q;
-  const int64_t ybeg = (int64_t)split * a.rows_per_split;
+  const int64_t ybeg = (int64_t)split * (a.split_stride > 0 ? a.split_stride : a.rows_per_split);
   const int64_t yend = ybeg + a.rows_per_split < a.n_t ? ybeg + a.rows_per_split : a.n_t;
   const size_t blk = (size_t)split * a.n_qblocks + qb;
   T* cand_d = a.cand_d + blk * kThreads * a.cap;
@@ -51,47 +65,68 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
   T xm2[DP];
 #pragma unroll
   for (int f = 0; f < DP; ++f) xm2[f] = (qok && f < a.d) ? T(-2) * a.X[q * a.d + f] : T(0);
-  T tau = qok ? max_finite<T>() : -pos_inf<T>();  // +inf marks rows past the end
+  // +inf marks rows past the end; lanes without a (selected) query take -inf.
+  const bool active = qok && (MODE != kLowdResearch || a.query_active[q] != 0);
+  T tau = active ? max_finite<T>() : -pos_inf<T>();
+  if (MODE != kLowdSample && active) {  // start from the best threshold known (sampled guess)
+    const T g = (T)from_ordered_key(load_agent(&a.tau[q]));
+    tau = g < tau ? g : tau;
+  }
+  float best[MODE == kLowdSample ? kSampleJ : 1];  // sample: kSampleJ smallest minima, ascending
+#pragma unroll
+  for (int i = 0; i < (MODE == kLowdSample ? kSampleJ : 1); ++i) best[i] = __builtin_huge_valf();
   cnt_s[tid] = 0;
 
-  // tile loader: rows x DP elements (zero padded) + norms (+inf past the end)
-  T pre[(kRows * DP + kThreads - 1) / kThreads];
+  // stage loader: SR rows x DP elements (zero padded) + norms (+inf past the end)
+  T pre[EPT];
   T pre_n = T(0);
   auto load_tile = [&](int64_t y0) {
 #pragma unroll
-    for (int i = 0; i < (kRows * DP + kThreads - 1) / kThreads; ++i) {
+    for (int i = 0; i < EPT; ++i) {
       const int e = tid + i * kThreads;
       const int r = e / DP, f = e % DP;
       const int64_t y = y0 + r;
-      pre[i] = (e < kRows * DP && y < yend && f < a.d) ? a.Y[y * a.d + f] : T(0);
+      pre[i] = (y < yend && f < a.d) ? a.Y[y * a.d + f] : T(0);
     }
-    if (tid < kRows) pre_n = (y0 + tid < yend) ? a.ynorm[y0 + tid] : pos_inf<T>();
+    if (tid < SR) pre_n = (y0 + tid < yend) ? a.ynorm[y0 + tid] : pos_inf<T>();
   };
   auto store_tile = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < (kRows * DP + kThreads - 1) / kThreads; ++i) {
+    for (int i = 0; i < EPT; ++i) {
       const int e = tid + i * kThreads;
-      if (e < kRows * DP) ys[buf][e / DP][e % DP] = pre[i];
+      ys[buf][e / DP][e % DP] = pre[i];
     }
-    if (tid < kRows) yn_s[buf][tid] = pre_n;
+    if (tid < SR) yn_s[buf][tid] = pre_n;
   };
   load_tile(ybeg);
   store_tile(0);
   __syncthreads();
 
   int cur = 0, step = 0;
-  for (int64_t y0 = ybeg; y0 < yend; y0 += kRows, ++step) {
-    const bool more = y0 + kRows < yend;
-    if (more) load_tile(y0 + kRows);
+  for (int64_t ys0 = ybeg; ys0 < yend; ys0 += SR) {
+    const bool more = ys0 + SR < yend;
+    if (more) load_tile(ys0 + SR);
+  for (int sub = 0; sub < SR && ys0 + sub < yend; sub += kRows, ++step) {
+    const int64_t y0 = ys0 + sub;
     T v[kRows];
     T m = pos_inf<T>();
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-      T acc = yn_s[cur][r];
+      T acc = yn_s[cur][sub + r];
 #pragma unroll
-      for (int f = 0; f < DP; ++f) acc = fma(xm2[f], ys[cur][r][f], acc);
+      for (int f = 0; f < DP; ++f) acc = fma(xm2[f], ys[cur][sub + r][f], acc);
       v[r] = acc;
       m = m < acc ? m : acc;
+    }
+    if constexpr (MODE == kLowdSample) {
+      // Insert m into the sorted run: b[i] <- min(b[i], max(b[i-1], m)).
+      const float mf = (float)m;  // a NaN drops this step's minimum
+      if (__any(qok && mf < best[kSampleJ - 1])) {
+#pragma unroll
+        for (int i = kSampleJ - 1; i > 0; --i) best[i] = fminf(best[i], fmaxf(best[i - 1], mf));
+        best[0] = fminf(best[0], mf);
+      }
+      continue;
     }
     bool overflow = false;
     if (__any(m <= tau)) {
@@ -129,9 +164,18 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
       const T g = (T)from_ordered_key(load_agent(&a.tau[q]));
       tau = g < tau ? g : tau;
     }
+  }
     if (more) store_tile(cur ^ 1);
     __syncthreads();
     cur ^= 1;
+  }
+  if constexpr (MODE == kLowdSample) {
+    if (qok) {
+      float* o = a.sample_out + ((size_t)q * a.n_splits + split) * kSampleJ;
+#pragma unroll
+      for (int i = 0; i < kSampleJ; ++i) o[i] = best[i];
+    }
+    return;
   }
 
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -147,6 +191,7 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
   for (int qq = 0; qq < 64; ++qq) {
     const int ql = w * 64 + qq;
     if (q0 + ql >= a.n_q) break;
+    if (MODE == kLowdResearch && a.query_active[q0 + ql] == 0) continue;  // uniform
     const int n = cnt_s[ql];
     for (int j = lane; j < a.kc; j += 64) {
       const size_t o = ((size_t)(q0 + ql) * a.n_splits + split) * a.kc + j;
@@ -156,17 +201,27 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
   }
 }
 
-template <class T, int DP>
+template <class T, int DP, int MODE>
 void launch_dp(const LowdArgs<T>& a, hipStream_t s) {
-  const size_t lds = (size_t)4 * a.cap * (sizeof(T) + sizeof(int));
+  const size_t lds = MODE == kLowdSample ? 0 : (size_t)4 * a.cap * (sizeof(T) + sizeof(int));
   const unsigned blocks = (unsigned)((a.n_qblocks * a.n_splits + 7) / 8 * 8);
-  hipLaunchKernelGGL((lowd_search_kernel<T, DP>), dim3(blocks), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL((lowd_search_kernel<T, DP, MODE>), dim3(blocks), dim3(kThreads), lds, s, a);
+}
+
+template <class T, int MODE>
+bool launch_mode(const LowdArgs<T>& a, hipStream_t s) {
+  if (a.d <= 4) launch_dp<T, 4, MODE>(a, s);
+  else if (a.d <= 8) launch_dp<T, 8, MODE>(a, s);
+  else if (a.d <= 16) launch_dp<T, 16, MODE>(a, s);
+  else return false;
+  return true;
 }
 
 template <class T, int DP>
 int occupancy_dp(int cap) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lowd_search_kernel<T, DP>, kThreads,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lowd_search_kernel<T, DP, kLowdSearch>,
+                                                   kThreads,
                                                    (size_t)4 * cap * (sizeof(T) + sizeof(int))) !=
           hipSuccess ||
       n <= 0)
@@ -178,11 +233,15 @@ int occupancy_dp(int cap) {
 
 template <class T>
 bool launch_lowd_search(const LowdArgs<T>& a, hipStream_t s) {
-  if (a.d <= 4) launch_dp<T, 4>(a, s);
-  else if (a.d <= 8) launch_dp<T, 8>(a, s);
-  else if (a.d <= 16) launch_dp<T, 16>(a, s);
-  else return false;
-  return true;
+  return launch_mode<T, kLowdSearch>(a, s);
+}
+
+bool launch_lowd_sample(const LowdArgs<float>& a, hipStream_t s) {
+  return a.sample_out && launch_mode<float, kLowdSample>(a, s);
+}
+
+bool launch_lowd_research(const LowdArgs<float>& a, hipStream_t s) {
+  return a.qblock_active && a.query_active && launch_mode<float, kLowdResearch>(a, s);
 }
 
 template <class T>
