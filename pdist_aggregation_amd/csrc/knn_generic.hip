@@ -23,26 +23,23 @@ namespace knn {
 
 namespace {
 
+// Sort and cut to k' every list of the workgroup longer than min_count: each
+// wave takes queries w, w+4, ... and sorts them alone (wave_sort_cut in its
+// own LDS slice sd/si + w * cap), so four lists are sorted at once and no
+// workgroup barrier runs inside a sort.
 template <class T>
 __device__ void compact_lists(const SearchArgs<T>& a, T* cd, int* ci, T* tau_s, int* cnt_s,
-                              int64_t q0, int min_count, T* sd, int* si) {
-  __syncthreads();
-  for (int qq = 0; qq < kGenQ; ++qq) {
-    const int n = cnt_s[qq];  // uniform: LDS value written before a barrier
+                              int64_t q0, int min_count, T* sd_all, int* si_all) {
+  __syncthreads();  // appends and counts visible
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  T* sd = sd_all + (size_t)w * a.cap;
+  int* si = si_all + (size_t)w * a.cap;
+  for (int qq = w; qq < kGenQ; qq += kThreads / 64) {
+    const int n = cnt_s[qq];  // uniform: LDS value written before the barrier
     if (n <= min_count) continue;
-    const int P = next_pow2(n);
-    for (int e = threadIdx.x; e < P; e += blockDim.x) {
-      const bool ok = e < n;
-      sd[e] = ok ? cd[(size_t)qq * a.cap + e] : pos_inf<T>();
-      si[e] = ok ? ci[(size_t)qq * a.cap + e] : INT_MAX;
-    }
-    block_bitonic_sort(sd, si, P);
-    const int m = n < a.kc ? n : a.kc;
-    for (int e = threadIdx.x; e < m; e += blockDim.x) {
-      cd[(size_t)qq * a.cap + e] = sd[e];
-      ci[(size_t)qq * a.cap + e] = si[e];
-    }
-    if (threadIdx.x == 0) {
+    wave_sort_cut(cd + (size_t)qq * a.cap, ci + (size_t)qq * a.cap, n, a.kc, sd, si, lane);
+    if (lane == 0) {
+      const int m = n < a.kc ? n : a.kc;
       cnt_s[qq] = m;
       if (m == a.kc) {
         const T kth = sd[a.kc - 1];
@@ -50,8 +47,8 @@ __device__ void compact_lists(const SearchArgs<T>& a, T* cd, int* ci, T* tau_s, 
         atomicMin(&a.tau[q0 + qq], ordered_key((double)kth));
       }
     }
-    __syncthreads();
   }
+  __syncthreads();  // counts and thresholds visible to the workgroup
 }
 
 template <class T>
@@ -61,9 +58,9 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
   __shared__ T tau_s[kGenQ];
   __shared__ int cnt_s[kGenQ];
   __shared__ int flag_s;
-  extern __shared__ __align__(16) unsigned char dyn_smem[];
+  extern __shared__ __align__(16) unsigned char dyn_smem[];  // 4 waves x cap x (T + int)
   T* sd = reinterpret_cast<T*>(dyn_smem);
-  int* si = reinterpret_cast<int*>(dyn_smem + (size_t)a.cap * sizeof(T));
+  int* si = reinterpret_cast<int*>(dyn_smem + (size_t)(kThreads / 64) * a.cap * sizeof(T));
 
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
@@ -163,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
 template <class T>
 void launch_generic_search(const SearchArgs<T>& a, hipStream_t s) {
   dim3 grid((unsigned)ceil_div(a.n_q, kGenQ), (unsigned)a.n_splits);
-  const size_t lds = (size_t)a.cap * (sizeof(T) + sizeof(int));
+  const size_t lds = (size_t)(kThreads / 64) * a.cap * (sizeof(T) + sizeof(int));
   hipLaunchKernelGGL(generic_search_kernel<T>, grid, dim3(kThreads), lds, s, a);
 }
 
