@@ -72,7 +72,10 @@ typedef __attribute__((address_space(3))) void lds_void;
 // Direct-to-LDS buffer loads (lane l's bytes land at lds_addr + size * l),
 // written out so that the compiler does not order its own LDS reads behind
 // them: the sweep waits for them itself (vmcnt, then a barrier).  M0 takes
-// the LDS address, one wait state before the load.
+// the LDS address, one wait state before the load.  The compiler warns that
+// it may not preserve a clobbered M0: nothing it generates for these kernels
+// keeps a value in M0 (gfx9 LDS instructions do not use it; the generated
+// code has no M0 reference outside these statements), so none is lost.
 __device__ inline unsigned lds_address(const void* p) {
   return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }

@@ -84,18 +84,18 @@ extern "C" int pdist_probe_mfma_tflops(int fmt, int per_cu, int iters, int reps,
     return -2;
   float* out = nullptr;
   if (hipMalloc(&out, (size_t)blocks * 256 * sizeof(float)) != hipSuccess) return -3;
-  hipEvent_t e0, e1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 0;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = -2;
   float ms = 0.f;
   if (rc == 0) {
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, 0, out, iters, 1e-3f);
-    hipEventRecord(e0, 0);
+    bool ok = hipEventRecord(e0, 0) == hipSuccess;
     for (int r = 0; r < reps; ++r)
       hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, 0, out, iters, 1e-3f);
-    hipEventRecord(e1, 0);
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
-        hipGetLastError() != hipSuccess)
+    ok = ok && hipEventRecord(e1, 0) == hipSuccess;
+    if (!ok || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipGetLastError() != hipSuccess)
       rc = -4;
   }
   if (rc == 0) {
@@ -104,8 +104,8 @@ extern "C" int pdist_probe_mfma_tflops(int fmt, int per_cu, int iters, int reps,
     *ms_per_launch = ms / reps;
     *tflops = flops / (*ms_per_launch * 1e-3) / 1e12;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(out);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(out);
   return rc;
 }
