@@ -265,10 +265,12 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
                                         std::to_string(knn::kMfmaMaxCap - knn::kMfmaCapSlack));
     p.path = precision == KNN_PRECISION_BF16 ? kPathMfmaBF16 : kPathMfmaFP8;
     p.dm = std::max(p.path == kPathMfmaBF16 ? 32 : 64, knn::next_pow2((int)dim));
-  } else if (!force_generic && elem == 4 && dim % 4 == 0 && dim >= 16 && dim <= 128 &&
+  } else if (!force_generic && elem == 4 && dim % 4 == 0 && dim >= 16 && dim <= 256 &&
              mfma_cap_ok) {
     p.path = kPathMfmaF32;
-    p.dm = std::max(32, knn::next_pow2((int)dim));
+    // 192 is a search width too (48 chunks of 16 bytes per row): d in
+    // (128, 192] would otherwise pad to 256 and spend up to half the MFMAs on zeros.
+    p.dm = dim <= 128 ? std::max(32, knn::next_pow2((int)dim)) : dim <= 192 ? 192 : 256;
   } else if (dim < 16 && lowd_cap <= knn::kMfmaMaxCap) {
     p.path = kPathLowd;
   }

@@ -122,6 +122,9 @@ struct TileGeom {
   static constexpr int R = NCH < 16 ? 16 / NCH : 1;
   static constexpr int MASK = (NCH < 16 ? NCH : 16) - 1;
   __device__ static int swz(int r) { return (r / R) & MASK; }
+  // j ^ swz stays in the row: NCH is a power of two below 16, else a
+  // multiple of 16 (the swizzle flips bits inside an aligned run of 16).
+  static_assert(NCH < 16 ? (NCH & (NCH - 1)) == 0 : NCH % 16 == 0, "row chunks");
 };
 constexpr int kNBuf = 3;
 
@@ -160,7 +163,7 @@ constexpr int kModeSample = 1;
 constexpr int kModeResearch = 2;
 
 template <int FMT, int DM, int QG, int MODE>
-__global__ __launch_bounds__(kThreads, FMT == kFmtF32 ? 3 : 2) void mfma_search_kernel(MfmaArgs a) {
+__global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) void mfma_search_kernel(MfmaArgs a) {
   using F = Fmt<FMT>;
   using G = TileGeom<FMT, DM>;
   constexpr int WQ = QG * 32;           // queries per wave
@@ -731,6 +734,8 @@ int mfma_blocks_per_cu(int fmt, int dm, int cap) {
   KNN_OCC_CASE(kFmtF32, 32)
   KNN_OCC_CASE(kFmtF32, 64)
   KNN_OCC_CASE(kFmtF32, 128)
+  KNN_OCC_CASE(kFmtF32, 192)
+  KNN_OCC_CASE(kFmtF32, 256)
   KNN_OCC_CASE(kFmtBF16, 32)
   KNN_OCC_CASE(kFmtBF16, 64)
   KNN_OCC_CASE(kFmtBF16, 128)
@@ -752,6 +757,8 @@ static bool launch_mode(const MfmaArgs& a, hipStream_t s) {
   KNN_MFMA_CASE(kFmtF32, 32)
   KNN_MFMA_CASE(kFmtF32, 64)
   KNN_MFMA_CASE(kFmtF32, 128)
+  KNN_MFMA_CASE(kFmtF32, 192)
+  KNN_MFMA_CASE(kFmtF32, 256)
   KNN_MFMA_CASE(kFmtBF16, 32)
   KNN_MFMA_CASE(kFmtBF16, 64)
   KNN_MFMA_CASE(kFmtBF16, 128)
