@@ -87,7 +87,7 @@ struct Plan {
   size_t off_scalars = 0, off_ynorm = 0, off_tau = 0, off_cand_d = 0, off_cand_i = 0,
          off_part_d = 0, off_part_i = 0, off_mu = 0, off_xq = 0, off_yq = 0, off_xnorm = 0,
          off_xerr = 0, off_xscale = 0, off_yscale = 0, off_slots = 0, off_sample = 0,
-         off_guess = 0, off_failed = 0, off_failed_qb = 0, total = 0;
+         off_guess = 0, off_failed = 0, off_failed_qb = 0, off_uncert = 0, total = 0;
 };
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
@@ -346,6 +346,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   p.off_scalars = take(sizeof(knn::DeviceScalars));
   p.off_ynorm = take((size_t)(n_train + knn::kNormPad) * celem);
   p.off_tau = take((size_t)nq1 * sizeof(unsigned long long));
+  p.off_uncert = take((size_t)nq1);
   const size_t ncand = (size_t)p.n_qblocks * p.n_splits * bq * p.cap;
   p.off_cand_d = take(ncand * celem);
   p.off_cand_i = take(ncand * sizeof(int));
@@ -646,6 +647,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.k = (int)k;
   ma.squared = (flags & KNN_FLAG_SQUARED) ? 1 : 0;
   ma.sc = sc;
+  ma.uncert = reinterpret_cast<unsigned char*>(ws + p.off_uncert);
   ma.dist_out = dist_out;
   ma.idx_out = idx_out;
   // With more splits than one merge workgroup takes, groups of split lists
@@ -688,6 +690,9 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       merge(rm);
     }
   }
+  // Every query's final merge (the first one, or the re-search's for a query
+  // whose guess failed) has set its flag.
+  if (stats) knn::launch_count_uncertified(ma.uncert, n_test, sc, stream);
   timer.mark(3);
 
   hipError_t e = hipGetLastError();

@@ -201,6 +201,10 @@ struct MergeArgs {
   double unit_roundoff;
   int acc_terms;
   DeviceScalars* sc;
+  // Per query, set by its final merge: 1 if not certified, else 0 (counted
+  // into sc->n_uncertified afterwards by launch_count_uncertified; one
+  // atomic per query on a single address costs ~40 ns each across the XCDs).
+  unsigned char* uncert;
   T* dist_out;        // n_q x k
   int64_t* idx_out;   // n_q x k
   // Sampled-threshold check (nullptr: off).  The search started from a
@@ -227,6 +231,9 @@ struct MergeArgs {
 
 template <class T>
 void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s);
+// sc->n_uncertified += number of nonzero flags in uncert[0, n).
+void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
+                              hipStream_t s);
 
 // Backward of the k-NN distances (knn_backward.hip).
 template <class T>

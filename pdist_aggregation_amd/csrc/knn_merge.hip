@@ -474,7 +474,34 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
       const double lower = sqrt(lo > 0.0 ? lo : 0.0) - ex - ey;
       certified = lower > sqrt(kth_s);
     }
-    if (!certified) atomicAdd(&a.sc->n_uncertified, 1u);
+    a.uncert[q] = certified ? 0 : 1;
+  }
+}
+
+// Sum of the per-query uncertified flags: 16 bytes per thread and pass, one
+// atomic per workgroup.
+__global__ __launch_bounds__(kThreads) void count_flags_kernel(const unsigned char* __restrict__ f,
+                                                               int64_t n, unsigned int* out) {
+  __shared__ unsigned int wsum[kThreads / 64];
+  unsigned int c = 0;
+  const int64_t n16 = n >> 4;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * kThreads) {
+    const uint4 v = reinterpret_cast<const uint4*>(f)[i];
+    // flags are 0 or 1: the byte sums of a word fit in its bytes
+    const unsigned int s = v.x + v.y + v.z + v.w;
+    c += (s & 0xff) + ((s >> 8) & 0xff) + ((s >> 16) & 0xff) + (s >> 24);
+  }
+  if (blockIdx.x == 0)
+    for (int64_t i = (n16 << 4) + threadIdx.x; i < n; i += kThreads) c += f[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int t = 0;
+    for (int i = 0; i < kThreads / 64; ++i) t += wsum[i];
+    if (t) atomicAdd(out, t);
   }
 }
 
@@ -523,6 +550,14 @@ void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s) {
   const int64_t blocks = a.pre_d ? a.n_q * a.n_groups : a.n_q;
   hipLaunchKernelGGL(merge_rerank_kernel<T>, dim3((unsigned)blocks), dim3(kMergeThreads), lds, s,
                      a);
+}
+
+void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
+                              hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>(64, ceil_div(ceil_div(n, 16), (int64_t)kThreads));
+  hipLaunchKernelGGL(count_flags_kernel, dim3((unsigned)std::max<int64_t>(1, blocks)),
+                     dim3(kThreads), 0, s, uncert, n, &sc->n_uncertified);
 }
 
 template void launch_fill_keys<float>(unsigned long long*, int64_t, unsigned long long, hipStream_t);
