@@ -124,7 +124,11 @@ int device_cus() {
 // Split count for the one-wave-of-workgroups kernels (MFMA, low-d): run time
 // is proportional to rounds(s) / s with rounds(s) = ceil(n_qblocks * s /
 // slots); take the smallest s within 4% of the best, since every split adds
-// its own warm-up and k' candidates to merge.  PDIST_KNN_SPLITS overrides.
+// its own warm-up and k' candidates to merge.  Counts past merge_lists(k')
+// need the two-level merge, which roughly doubles the merge: they are only
+// considered when the one-level range is modelled more than 25% slower
+// (few query blocks; 1e5 queries at k' = 128: 17 splits instead of 11 gave
+// no faster search and 2.7 -> 6.2 ms of merge).  PDIST_KNN_SPLITS overrides.
 int choose_splits_slots(int64_t n_qblocks, int64_t n_train, int row_step, int kc,
                         int64_t slots) {
   if (const char* env = std::getenv("PDIST_KNN_SPLITS")) {
@@ -136,8 +140,18 @@ int choose_splits_slots(int64_t n_qblocks, int64_t n_train, int row_step, int kc
   auto cost = [&](int64_t s) {
     return (double)knn::ceil_div(n_qblocks * s, std::max<int64_t>(slots, 1)) / (double)s;
   };
-  double best = cost(1);
-  for (int64_t s = 2; s <= smax; ++s) best = std::min(best, cost(s));
+  auto best_upto = [&](int64_t hi) {
+    double b = cost(1);
+    for (int64_t s = 2; s <= hi; ++s) b = std::min(b, cost(s));
+    return b;
+  };
+  double best = best_upto(smax);
+  const int64_t s1 = std::min<int64_t>(smax, merge_lists(kc));
+  const double best1 = best_upto(s1);
+  if (best1 <= 1.25 * best) {
+    smax = s1;
+    best = best1;
+  }
   for (int64_t s = 1; s <= smax; ++s)
     if (cost(s) <= 1.04 * best) return (int)s;
   return 1;
