@@ -364,51 +364,81 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
   __syncthreads();  // od is overwritten below (it may share storage with ed)
 
   // Exact FP64 direct-difference distances of the best k' candidates: tpc
-  // consecutive threads per candidate, each over every tpc-th feature (so
-  // every candidate row's loads are in flight together), then a shuffle
-  // reduction over the tpc lanes.
+  // consecutive lanes per candidate (8 or 16: a load instruction reads whole
+  // 128-byte lines of each row; 16 when that already covers k' in one pass),
+  // kU candidates per lane group and pass with all their loads issued
+  // together, then a shuffle reduction over the tpc lanes.  A slot without a
+  // valid row reads row 0 (no divergent load) and is dropped.
+  constexpr int kU = 4;
   const T* xr = a.X + q * a.d;
-  const int tpc = KP >= kMergeThreads ? 1
-                  : (kMergeThreads / KP < 64 ? kMergeThreads / KP : 64);  // 2^i <= 64
   using V4 = typename Vec4<T>::type;
   const bool vec4 = (a.d & 3) == 0 &&
                     ((reinterpret_cast<uintptr_t>(a.X) | reinterpret_cast<uintptr_t>(a.Y)) &
                      (4 * sizeof(T) - 1)) == 0;
-  const int per_pass = kMergeThreads / tpc;
-  for (int j0 = 0; j0 < KP; j0 += per_pass) {
-    const int j = j0 + tid / tpc, c = tid % tpc;
-    const int cid = j < a.kc ? oi[j] : INT_MAX;
-    // Every slot below min(n, k') holds a staged row when keys are distinct;
-    // the range test keeps the gather inside Y whatever the slot holds.
-    const bool valid = cid >= 0 && cid < a.n_t;
-    double s = 0.0;
-    if (valid) {
-      const T* yr = a.Y + (int64_t)cid * a.d;
-      if (vec4) {  // 16-byte loads, every one of them in flight at once
-        const int d4 = a.d >> 2;
-#pragma unroll 8
-        for (int p = c; p < d4; p += tpc) {
-          const V4 xv = reinterpret_cast<const V4*>(xr)[p];
-          const V4 yv = reinterpret_cast<const V4*>(yr)[p];
-          const double d0 = (double)xv.x - (double)yv.x, d1 = (double)xv.y - (double)yv.y;
-          const double d2 = (double)xv.z - (double)yv.z, d3 = (double)xv.w - (double)yv.w;
-          s = fma(d0, d0, s);
-          s = fma(d1, d1, s);
-          s = fma(d2, d2, s);
-          s = fma(d3, d3, s);
+  const int span = vec4 ? (a.d >> 2) : a.d;  // loads per row
+  const int cover = kMergeThreads * kU / KP;  // lanes per candidate for one pass
+  int tpc = cover < 8 ? 8 : (cover > 16 ? 16 : cover);
+  while (tpc > 1 && tpc / 2 >= span) tpc >>= 1;  // no more lanes than loads
+  const int groups = kMergeThreads / tpc;
+  const int grp_j = tid / tpc, c = tid % tpc;
+  for (int j0 = 0; j0 < KP; j0 += groups * kU) {
+    int cid[kU];
+    bool valid[kU];
+    const T* yr[kU];
+    double s[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int j = j0 + u * groups + grp_j;
+      cid[u] = j < a.kc ? oi[j] : INT_MAX;
+      // Every slot below min(n, k') holds a staged row when keys are
+      // distinct; the range test keeps the gather inside Y whatever it holds.
+      valid[u] = cid[u] >= 0 && cid[u] < a.n_t;
+      yr[u] = a.Y + (int64_t)(valid[u] ? cid[u] : 0) * a.d;
+      s[u] = 0.0;
+    }
+    if (vec4) {
+#pragma unroll 2
+      for (int p = c; p < span; p += tpc) {
+        const V4 xv = reinterpret_cast<const V4*>(xr)[p];
+        V4 yv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) yv[u] = reinterpret_cast<const V4*>(yr[u])[p];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const double d0 = (double)xv.x - (double)yv[u].x, d1 = (double)xv.y - (double)yv[u].y;
+          const double d2 = (double)xv.z - (double)yv[u].z, d3 = (double)xv.w - (double)yv[u].w;
+          s[u] = fma(d0, d0, s[u]);
+          s[u] = fma(d1, d1, s[u]);
+          s[u] = fma(d2, d2, s[u]);
+          s[u] = fma(d3, d3, s[u]);
         }
-      } else {
-        for (int p = c; p < a.d; p += tpc) {
-          const double df = (double)xr[p] - (double)yr[p];
-          s = fma(df, df, s);
+      }
+    } else {
+      for (int p = c; p < span; p += tpc) {
+        const double x = (double)xr[p];
+        T yv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) yv[u] = yr[u][p];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const double df = x - (double)yv[u];
+          s[u] = fma(df, df, s[u]);
         }
       }
     }
-    for (int o = tpc >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      for (int o = tpc >> 1; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o, 64);
     __syncthreads();  // every oi[] read of this pass precedes the ed/ei writes (shared storage)
-    if (c == 0 && j < KP) {
-      ed[j] = valid ? s : __builtin_huge_val();
-      ei[j] = valid ? cid : INT_MAX;
+    if (c == 0) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int j = j0 + u * groups + grp_j;
+        if (j < KP) {
+          ed[j] = valid[u] ? s[u] : __builtin_huge_val();
+          ei[j] = valid[u] ? cid[u] : INT_MAX;
+        }
+      }
     }
   }
   if (w == 0) {
