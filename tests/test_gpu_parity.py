@@ -229,6 +229,19 @@ def test_f32_kernel_path_selection(require_gpu, d, expect_path):
     assert st["n_uncertified"] == 0
 
 
+@pytest.mark.parametrize("d,k", [(256, 100), (256, 400), (192, 250)])
+def test_mfma_f32_wide_rows_large_k(require_gpu, d, k):
+    """float32 at search widths 192 / 256 with lists near their largest size."""
+    rs = np.random.RandomState(d + k)
+    Y = rs.rand(12000, d).astype(np.float32)
+    X = rs.rand(300, d).astype(np.float32)
+    (dist, idx), _, st = pa.parallel_knn(Y, X, k, return_distance=True, squared=True,
+                                         return_stats=True)
+    assert st["kernel_path"] == 1
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, k, EPS_F32)
+
+
 @pytest.mark.parametrize("k", [1, 10, 100, 400])
 def test_mfma_f32_many_k(require_gpu, k):
     rs = np.random.RandomState(k)

@@ -49,6 +49,8 @@ def _check(X, Y, dist, idx, k, precision, st):
 
 CASES = [("exact", 64, 10), ("exact", 128, 10), ("exact", 32, 1), ("bf16", 128, 10),
          ("bf16", 50, 32), ("fp8", 128, 10),
+         # float32 search widths 192 and 256 (one or two workgroups per CU)
+         ("exact", 160, 10), ("exact", 256, 10),
          # low-d kernel (one lane per query, padded to 4 / 8 / 16 features)
          ("exact", 3, 10), ("exact", 7, 16), ("exact", 12, 5)]
 
