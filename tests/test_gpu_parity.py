@@ -78,7 +78,7 @@ def test_f32_matches_oracle(require_gpu, name):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("d", [1, 2, 3, 7, 16, 50, 128, 129, 300])
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 7, 8, 12, 16, 50, 128, 129, 300])
 def test_dims_and_ragged_sizes(require_gpu, dtype, d):
     rs = np.random.RandomState(d)
     Y = rs.rand(1237, d).astype(dtype)
