@@ -123,3 +123,18 @@ def test_rate_probe_library_loads_and_rejects_bad_arguments():
     assert fn(0, 0, 10, 1, ctypes.byref(tf), ctypes.byref(ms)) == -1  # per_cu out of range
     assert fn(0, 5, 10, 1, ctypes.byref(tf), ctypes.byref(ms)) == -1
     assert fn(0, 1, 0, 1, ctypes.byref(tf), ctypes.byref(ms)) == -1  # no iterations
+
+
+def test_workload_inputs_cpu():
+    """The timed workloads build the inputs the GPU tests check (CPU only)."""
+    from pdist_aggregation_amd import workloads
+
+    Y, X, k, precision, over = workloads._inputs("plain-f64")
+    assert Y.shape == (10_000, 50) and X.shape == (1_000, 50) and Y.dtype == np.float64
+    assert (k, precision, over) == (10, "exact", 0)
+    Y, X, k, precision, over = workloads._inputs("lowd")
+    assert Y.shape == X.shape == (65536, 3) and Y.dtype == np.float32 and k == 16
+    r = np.linalg.norm(X.astype(np.float64), axis=1)
+    assert abs(r.mean() - 1.0) < 1e-3 and 0.005 < r.std() < 0.02
+    with pytest.raises(ValueError):
+        workloads._inputs("nope")
