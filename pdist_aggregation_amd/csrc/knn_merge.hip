@@ -52,10 +52,20 @@ __global__ __launch_bounds__(kThreads) void sample_guess_kernel(
     if (q % qblock_queries == 0) failed_qblocks[q / qblock_queries] = 0;
   }
   float b[kSampleJ];
-#pragma unroll
-  for (int i = 0; i < kSampleJ; ++i) b[i] = __builtin_huge_valf();
   const float* p = so + (size_t)q * n_runs * kSampleJ;
-  for (int r = lane; r < n_runs; r += 64) {
+  // A lane's first run is its sorted list as it stands (runs are ascending);
+  // later runs (more than 64 per query) are merged into it from registers.
+#pragma unroll
+  for (int i = 0; i < kSampleJ; i += 4) {
+    f4 v = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf(),
+            __builtin_huge_valf()};
+    if (lane < n_runs) v = *reinterpret_cast<const f4*>(p + (size_t)lane * kSampleJ + i);
+    b[i] = v[0];
+    b[i + 1] = v[1];
+    b[i + 2] = v[2];
+    b[i + 3] = v[3];
+  }
+  for (int r = lane + 64; r < n_runs; r += 64) {
     // The whole run in four 16-byte loads, then merged from registers.
     float run[kSampleJ];
 #pragma unroll
