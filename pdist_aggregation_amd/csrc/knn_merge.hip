@@ -163,9 +163,12 @@ struct Vec4<double> {
 constexpr int kRankMax = 128;
 // k' up to which the exact distances are ordered by ranking.
 constexpr int kRankExact = 256;
-// Merge workgroup: two waves per query, so twice as many queries are in
-// flight per CU as with four (each phase holds at most a few entries per
-// thread, so the smaller group does not lengthen a query).
+// Merge workgroup: two waves per query (kMaxMerge entries, kPerThread per
+// thread), or one wave when the query's entries fit kPerThread per lane of
+// one: each phase holds at most a few entries per thread, so the smaller
+// group does not lengthen a query and twice as many are in flight per CU
+// (65,536 low-d queries x 5 splits x k'=24: merge 0.41 -> 0.21 ms; float32
+// 1e4 x 19 x 18: 0.084 -> 0.056 ms; larger merges are held by LDS instead).
 constexpr int kMergeThreads = 128;
 // Split-list entries per thread held in registers (kMaxMerge / kMergeThreads).
 constexpr int kPerThread = kMaxMerge / kMergeThreads;
@@ -214,10 +217,11 @@ __device__ inline int count_less(const T* kd, const int* ki, int n, T x, int xi)
   return r;
 }
 
-// One workgroup per query (final merge) or per (query, group of splits)
-// (pre-merge, a.pre_d set).
-template <class T>
-__global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T> a) {
+// One workgroup of NT threads per query (final merge) or per (query, group of
+// splits) (pre-merge, a.pre_d set); the launch guarantees lists * k' <= NT *
+// kPerThread.
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
   extern __shared__ __align__(16) unsigned char dyn_smem[];
   const bool pre = a.pre_d != nullptr;
   const int64_t q = pre ? (int64_t)blockIdx.x / a.n_groups : (int64_t)blockIdx.x;
@@ -260,19 +264,19 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kPerThread; ++i) {
-    const int e = tid + i * kMergeThreads;
+    const int e = tid + i * NT;
     const bool in = e < total;
     id[i] = in ? pi[e] : -1;
     v[i] = in ? pd[e] : pos_inf<T>();
     if (!(id[i] >= 0 && id[i] < a.n_t && v[i] <= tq)) id[i] = -1;  // never index past Y
   }
-  const int rounds = (total + kMergeThreads - 1) / kMergeThreads;  // uniform
+  const int rounds = (total + NT - 1) / NT;  // uniform
   {
     bool head[kPerThread];
     int at[kPerThread];
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i) {
-      const int e = tid + i * kMergeThreads;
+      const int e = tid + i * NT;
       head[i] = id[i] >= 0 && e % a.kc < r;
     }
     wave_slots(head, rounds, &n_s, at);
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
   if (tid == 0) cut_s = tq;
   __syncthreads();
   if (nh >= a.kc) {
-    for (int e = tid; e < nh; e += kMergeThreads) {
+    for (int e = tid; e < nh; e += NT) {
       const T x = sd[e];
       const int xi = si[e];
       const int rank = count_less(sd, si, nh, x, xi);
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
   if (n <= kRankMax) {
     // (value, index) keys are distinct (a train row is in one split only), so
     // ranks are a permutation.
-    for (int e = tid; e < n; e += kMergeThreads) {
+    for (int e = tid; e < n; e += NT) {
       const T x = sd[e];
       const int xi = si[e];
       const int rank = count_less(sd, si, n, x, xi);
@@ -332,20 +336,20 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
     }
   } else {
     const int Pn = next_pow2(n);
-    for (int e = n + tid; e < Pn; e += kMergeThreads) {
+    for (int e = n + tid; e < Pn; e += NT) {
       sd[e] = pos_inf<T>();
       si[e] = INT_MAX;
     }
     block_bitonic_sort(sd, si, Pn);
   }
-  for (int j = n + tid; j < a.kc; j += kMergeThreads) {
+  for (int j = n + tid; j < a.kc; j += NT) {
     od[j] = pos_inf<T>();
     oi[j] = INT_MAX;
   }
   __syncthreads();
   if (pre) {  // the group's k' best, sorted, for the final merge
     const size_t o = ((size_t)q * a.n_groups + grp) * a.kc;
-    for (int j = tid; j < a.kc; j += kMergeThreads) {
+    for (int j = tid; j < a.kc; j += NT) {
       a.pre_d[o + j] = od[j];
       a.pre_i[o + j] = oi[j] == INT_MAX ? -1 : oi[j];
     }
@@ -386,10 +390,10 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
                     ((reinterpret_cast<uintptr_t>(a.X) | reinterpret_cast<uintptr_t>(a.Y)) &
                      (4 * sizeof(T) - 1)) == 0;
   const int span = vec4 ? (a.d >> 2) : a.d;  // loads per row
-  const int cover = kMergeThreads * kU / KP;  // lanes per candidate for one pass
+  const int cover = NT * kU / KP;  // lanes per candidate for one pass
   int tpc = cover < 8 ? 8 : (cover > 16 ? 16 : cover);
   while (tpc > 1 && tpc / 2 >= span) tpc >>= 1;  // no more lanes than loads
-  const int groups = kMergeThreads / tpc;
+  const int groups = NT / tpc;
   const int grp_j = tid / tpc, c = tid % tpc;
   for (int j0 = 0; j0 < KP; j0 += groups * kU) {
     int cid[kU];
@@ -467,12 +471,12 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
   // of it (distinct keys); more: bitonic sort.
   if (a.kc <= kRankExact) {
     if (tid == 0) kth_s = __builtin_huge_val();
-    for (int j = tid; j < a.k; j += kMergeThreads) {  // slots no candidate fills
+    for (int j = tid; j < a.k; j += NT) {  // slots no candidate fills
       a.idx_out[q * a.k + j] = -1;
       a.dist_out[q * a.k + j] = pos_inf<T>();
     }
     __syncthreads();
-    for (int j = tid; j < a.kc; j += kMergeThreads) {
+    for (int j = tid; j < a.kc; j += NT) {
       if (ei[j] == INT_MAX) continue;
       const double x = ed[j];
       const int xi = ei[j];
@@ -485,7 +489,7 @@ __global__ __launch_bounds__(kMergeThreads) void merge_rerank_kernel(MergeArgs<T
     }
   } else {
     block_bitonic_sort(ed, ei, KP);
-    for (int j = tid; j < a.k; j += kMergeThreads) {
+    for (int j = tid; j < a.k; j += NT) {
       const bool ok = j < a.kc && ei[j] != INT_MAX;
       const double x = ok ? ed[j] : __builtin_huge_val();
       a.idx_out[q * a.k + j] = ok ? (int64_t)ei[j] : -1;
@@ -590,8 +594,12 @@ void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s) {
   if (a.n_q <= 0) return;
   const size_t lds = merge_lds_bytes(a.pre_d ? a.group_size : a.n_splits, a.kc, sizeof(T));
   const int64_t blocks = a.pre_d ? a.n_q * a.n_groups : a.n_q;
-  hipLaunchKernelGGL(merge_rerank_kernel<T>, dim3((unsigned)blocks), dim3(kMergeThreads), lds, s,
-                     a);
+  const int lists = a.pre_d ? a.group_size : a.n_splits;
+  if (lists * a.kc <= 64 * kPerThread)
+    hipLaunchKernelGGL((merge_rerank_kernel<T, 64>), dim3((unsigned)blocks), dim3(64), lds, s, a);
+  else
+    hipLaunchKernelGGL((merge_rerank_kernel<T, kMergeThreads>), dim3((unsigned)blocks),
+                       dim3(kMergeThreads), lds, s, a);
 }
 
 void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
