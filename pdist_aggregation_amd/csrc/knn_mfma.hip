@@ -67,8 +67,6 @@ struct Fmt<kFmtFP8> {
   using breg = i32x8;
 };
 
-typedef __attribute__((address_space(3))) void lds_void;
-
 // Direct-to-LDS buffer loads (lane l's bytes land at lds_addr + size * l),
 // written out so that the compiler does not order its own LDS reads behind
 // them: the sweep waits for them itself (vmcnt, then a barrier).  M0 takes
