@@ -120,6 +120,19 @@ def test_k_equals_n_train_and_single_query(require_gpu, dtype):
     assert idx[0, 0] == bi[0, 0]
 
 
+@pytest.mark.parametrize("dtype,precision,d", [(np.float32, "exact", 64), (np.float32, "exact", 3),
+                                               (np.float64, "exact", 50),
+                                               (np.float32, "bf16", 128),
+                                               (np.float32, "fp8", 128)])
+def test_no_queries(require_gpu, dtype, precision, d):
+    """An empty query set returns empty (0, k) results on every path."""
+    rs = np.random.RandomState(1)
+    Y = rs.rand(500, d).astype(dtype)
+    X = np.empty((0, d), dtype=dtype)
+    (dist, idx), _ = pa.parallel_knn(Y, X, 7, return_distance=True, precision=precision)
+    assert dist.shape == (0, 7) and idx.shape == (0, 7)
+
+
 def test_large_k(require_gpu):
     rs = np.random.RandomState(9)
     Y = rs.rand(3000, 10)
