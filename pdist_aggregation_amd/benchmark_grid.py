@@ -140,7 +140,7 @@ def main(argv=None):
 
     with open(args.config) as f:
         config = yaml.safe_load(f)
-    folder = args.results or f"gpurun_out/benchmarks/{args.bench_name}"
+    folder = args.results or f"benchmark_results/{args.bench_name}"
     benchmark(config, folder, args.bench_name)
     print(f"Benchmark results wrote in {folder}")
 
