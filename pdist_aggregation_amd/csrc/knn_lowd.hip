@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
   float best[MODE == kLowdSample ? kSampleJ : 1];  // sample: kSampleJ smallest minima, ascending
 #pragma unroll
   for (int i = 0; i < (MODE == kLowdSample ? kSampleJ : 1); ++i) best[i] = __builtin_huge_valf();
-  cnt_s[tid] = 0;
+  int cnt = 0;  // entries in this lane's list (published in cnt_s for wave-wide passes)
 
   // stage loader: SR rows x DP elements (zero padded) + norms (+inf past the end)
   T pre[EPT];
@@ -122,8 +122,11 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
 #pragma unroll
       for (int f = NIR ? 1 : 0; f < DP; ++f) acc = fma(xm2[f], ys[cur][sub + r][f], acc);
       v[r] = acc;
-      m = m < acc ? m : acc;
     }
+    // min tree of 3-input minima (v_min3); a NaN value is passed over, so it
+    // hides no other row of the step (and is itself never appended).
+#pragma unroll
+    for (int r = 0; r + 1 < kRows; r += 2) m = fmin(m, fmin(v[r], v[r + 1]));
     if constexpr (MODE == kLowdSample) {
       // Insert m into the sorted run: b[i] <- min(b[i], max(b[i-1], m)).
       const float mf = (float)m;  // a NaN drops this step's minimum
@@ -137,18 +140,21 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
     bool overflow = false;
     if (__any(m <= tau)) {
       if (m <= tau) {
+        int pos = cnt;  // this lane owns the query: no atomic
 #pragma unroll
         for (int r = 0; r < kRows; ++r) {
           if (v[r] <= tau) {
-            const int pos = cnt_s[tid]++;  // this lane owns the query: no atomic
             my_d[pos] = v[r];
             my_i[pos] = (int)(y0 + r);
+            ++pos;
           }
         }
-        overflow = cnt_s[tid] > a.cap - kRows;
+        cnt = pos;
+        overflow = pos > a.cap - kRows;
       }
     }
     if (__any(overflow)) {
+      cnt_s[tid] = cnt;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       wave_sync();
       for (int qq = 0; qq < 64; ++qq) {
@@ -165,6 +171,8 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
         if (lane == qq && n >= a.kc) tau = tau < sd[a.kc - 1] ? tau : sd[a.kc - 1];
         wave_sync();
       }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      cnt = cnt_s[tid];
     }
     if ((step & 15) == 15 && qok) {
       const T g = (T)from_ordered_key(load_agent(&a.tau[q]));
@@ -184,6 +192,7 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
     return;
   }
 
+  cnt_s[tid] = cnt;
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   wave_sync();
   for (int qq = 0; qq < 64; ++qq) {
