@@ -98,22 +98,6 @@ constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP
 int64_t merge_lists(int kc) { return std::max(1, knn::kMaxMerge / kc); }
 int64_t merge_split_limit(int kc) { return merge_lists(kc) * merge_lists(kc); }
 
-// Split count for a grid of n_qblocks query blocks: enough workgroups to keep
-// every CU busy several times over, each split at least 8 row steps -- or 4
-// while that stays within one merge level (1e4 x 1e3 x 50 float64: 18 -> 32
-// splits, 0.68 -> 0.52 ms) --, within what the two-level merge takes, and
-// (MFMA) a multiple of 8 so a split's query blocks share one XCD.
-int choose_splits(int64_t n_qblocks, int64_t n_train, int row_step, int kc, int64_t target,
-                  bool multiple_of_8) {
-  int64_t splits = std::max<int64_t>(1, (target + n_qblocks / 2) / n_qblocks);
-  const int64_t s8 = std::max<int64_t>(1, n_train / (8 * row_step));
-  const int64_t s4 = std::max<int64_t>(1, n_train / (4 * row_step));
-  splits = std::min<int64_t>(splits, std::min<int64_t>(s4, std::max<int64_t>(s8, merge_lists(kc))));
-  splits = std::min<int64_t>(splits, merge_split_limit(kc));
-  if (multiple_of_8 && splits >= 8) splits = splits / 8 * 8;
-  return (int)std::max<int64_t>(splits, 1);
-}
-
 // Compute units of the current device (256 on an MI355X; also the fallback
 // when no device is visible, so plans made without a GPU match one).
 int device_cus() {
@@ -122,6 +106,24 @@ int device_cus() {
       hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
     return n;
   return 256;
+}
+
+// Split count for a grid of n_qblocks query blocks: enough workgroups to keep
+// every CU busy several times over, each split at least 8 row steps -- or 2
+// when 8 would leave the CUs with fewer than two workgroups each (1e3 x 1e4 x
+// 50 float64, 16 query blocks: 18 -> 53 splits, k=10 0.69 -> 0.52 ms, k=100
+// 1.33 -> 0.83 ms, even with the two-level merge) --, within what the
+// two-level merge takes, and (MFMA) a multiple of 8 so a split's query blocks
+// share one XCD.
+int choose_splits(int64_t n_qblocks, int64_t n_train, int row_step, int kc, int64_t target,
+                  bool multiple_of_8) {
+  int64_t splits = std::max<int64_t>(1, (target + n_qblocks / 2) / n_qblocks);
+  const int64_t s8 = std::max<int64_t>(1, n_train / (8 * row_step));
+  const bool thin = n_qblocks * s8 < 2 * (int64_t)device_cus();
+  splits = std::min<int64_t>(splits, thin ? std::max<int64_t>(1, n_train / (2 * row_step)) : s8);
+  splits = std::min<int64_t>(splits, merge_split_limit(kc));
+  if (multiple_of_8 && splits >= 8) splits = splits / 8 * 8;
+  return (int)std::max<int64_t>(splits, 1);
 }
 
 // Split count for the one-wave-of-workgroups kernels (MFMA, low-d): run time
