@@ -111,32 +111,17 @@ __global__ __launch_bounds__(kThreads) void quantize_bf16_kernel(
 }
 
 // OCP FP8 E4M3 (bias 7, no infinities, max normal 448), round to nearest
-// even, in software so the encoding does not depend on a conversion
-// instruction.  |v| <= 448 on entry.
+// even, with the gfx950 conversion instructions (v_cvt_pk_fp8_f32 /
+// v_cvt_f32_fp8).  |v| <= 448 on entry.  Their bytes equal those of a
+// software round-to-nearest-even encoder (frexp, 3-bit mantissa, subnormals
+// m * 2^-9, saturation at 448) for every exponent from 2^-20 to 448 and all
+// 16-bit rounding patterns below the kept bits (3.9e6 values checked on the
+// GPU), except that -0 keeps its sign bit (0x80, the value is the same).
 __device__ inline unsigned encode_e4m3(float v) {
-  const unsigned sign = v < 0.f ? 0x80u : 0u;
-  const float a = fabsf(v);
-  if (a < 0x1p-6f) {  // subnormal: m * 2^-9, m in 0..7 (8 rounds up to the min normal)
-    const unsigned m = (unsigned)rintf(a * 512.f);
-    return sign | m;  // m == 8 encodes exponent 1, mantissa 0: the min normal
-  }
-  int e;
-  const float fr = frexpf(a, &e);  // a = fr * 2^e, fr in [0.5, 1)
-  unsigned m = (unsigned)rintf((fr * 2.f - 1.f) * 8.f);
-  int ex = e - 1 + 7;
-  if (m == 8) {
-    m = 0;
-    ++ex;
-  }
-  if (ex > 15 || (ex == 15 && m > 6)) return sign | 0x7eu;  // saturate at 448
-  return sign | ((unsigned)ex << 3) | m;
+  return (unsigned)__builtin_amdgcn_cvt_pk_fp8_f32(v, 0.f, 0, false) & 0xffu;
 }
 
-__device__ inline float decode_e4m3(unsigned b) {
-  const unsigned ex = (b >> 3) & 15u, m = b & 7u;
-  const float a = ex == 0 ? (float)m * 0x1p-9f : ldexpf(1.f + (float)m * 0.125f, (int)ex - 7);
-  return (b & 0x80u) ? -a : a;
-}
+__device__ inline float decode_e4m3(unsigned b) { return __builtin_amdgcn_cvt_f32_fp8((int)b, 0); }
 
 constexpr int kFp8MaxPerLane = 4;  // dm <= 256
 
@@ -381,14 +366,22 @@ __global__ __launch_bounds__(kThreads) void quantize_fp8_vec_kernel(
         e = fr > 0.5f ? E : E - 1;                   // smallest e with amax/448 <= 2^e
         e = e < -120 ? -120 : (e > 120 ? 120 : e);
       }
-      unsigned word = 0;
       float ns = 0.f;
       double es = 0.0;
+      // four features in two packed conversions: bytes 0, 1 (low half), 2, 3
+      int packed = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(cen[0], -e), ldexpf(cen[1], -e), 0,
+                                                   false);
+      packed = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(cen[2], -e), ldexpf(cen[3], -e), packed,
+                                               true);
+      if (!fok) packed = 0;
+      const unsigned word = (unsigned)packed;
+      const float dq[4] = {__builtin_amdgcn_cvt_f32_fp8(packed, 0),
+                           __builtin_amdgcn_cvt_f32_fp8(packed, 1),
+                           __builtin_amdgcn_cvt_f32_fp8(packed, 2),
+                           __builtin_amdgcn_cvt_f32_fp8(packed, 3)};
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const unsigned b = fok ? encode_e4m3(ldexpf(cen[i], -e)) : 0u;
-        word |= b << (8 * i);
-        const float q = ldexpf(decode_e4m3(b), e);
+        const float q = ldexpf(dq[i], e);
         ns = fmaf(q, q, ns);
         const double ed = fok ? ((double)xv[i] - (double)mv[i]) - (double)q : 0.0;
         es = fma(ed, ed, es);
