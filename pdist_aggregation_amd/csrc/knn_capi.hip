@@ -188,6 +188,13 @@ double binom_tail(int n, double f, int j) {
 // every S-th query forced to fail (exercises the re-search; "fail" = every
 // query).
 constexpr int64_t kGuessMinRows = 32768;
+// Generic kernel: workgroups its search grid aims for, and the list capacity
+// of its sampling launch (kSampleJ kept + one 64-row step of appends).
+constexpr int64_t kGenericGridTarget = 2048;
+constexpr int kGenericSampleCap = 128;
+static_assert(kGenericSampleCap >= knn::kSampleJ + knn::kGenY &&
+                  (kGenericSampleCap & (kGenericSampleCap - 1)) == 0,
+              "sampling lists: a power of two holding kSampleJ plus one step");
 constexpr int kSampleShift = 5;  // sample 1/32 of the train rows
 constexpr double kGuessFail = 1e-7;
 constexpr int kGuessMaxJ = 128;
@@ -318,7 +325,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   int splits;
   int64_t slots = 0;
   if (p.path == kPathGeneric) {
-    splits = choose_splits(p.n_qblocks, n_train, step, p.kc, 2048, false);
+    splits = choose_splits(p.n_qblocks, n_train, step, p.kc, kGenericGridTarget, false);
   } else {
     int per_cu = 2;
     if (p.path == kPathLowd)
@@ -353,6 +360,12 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   const bool mfma = p.path == kPathMfmaF32 || p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8;
   if (mfma) plan_guess(p, slots, 2);
   if (p.path == kPathLowd && elem == 4) plan_guess(p, slots, 1);
+  // Generic kernel: one run per (query, region), from as many sampling
+  // workgroups as the search's own grid target.  Only with 4 or more splits:
+  // the guess saves each split's first fill and compaction, and with fewer
+  // the sampling launch costs more than those (float64 1e5 queries x 1e3
+  // rows x 100, k = 10, one split: 2.6 -> 2.9 ms with it).
+  if (p.path == kPathGeneric && p.n_splits >= 4) plan_guess(p, kGenericGridTarget, 1);
 
   const size_t celem = (p.path == kPathGeneric || p.path == kPathLowd) ? elem : sizeof(float);
   const int64_t nq1 = std::max<int64_t>(n_test, 1);
@@ -366,7 +379,9 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   p.off_ynorm = take((size_t)(n_train + knn::kNormPad) * celem);
   p.off_tau = take((size_t)nq1 * sizeof(unsigned long long));
   p.off_uncert = take((size_t)nq1);
-  const size_t ncand = (size_t)p.n_qblocks * p.n_splits * bq * p.cap;
+  size_t ncand = (size_t)p.n_qblocks * p.n_splits * bq * p.cap;
+  if (p.guess && p.path == kPathGeneric)  // the sampling launch's lists share this scratch
+    ncand = std::max(ncand, (size_t)p.n_qblocks * p.sample_regions * bq * kGenericSampleCap);
   p.off_cand_d = take(ncand * celem);
   p.off_cand_i = take(ncand * sizeof(int));
   const size_t npart = (size_t)nq1 * p.n_splits * p.kc;
@@ -514,10 +529,29 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     sa.tau = tau;
     sa.part_d = reinterpret_cast<T*>(ws + p.off_part_d);
     sa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
-    // 2b. sampled threshold (low-d, float): minima over 1/32 of the rows
-    // -> guess -> tau, as for the MFMA kernels (one run per query and region).
+    // 2b. sampled threshold: the kSampleJ smallest over 1/32 of the rows
+    // (generic; low-d float: of 32-row minima) -> guess -> tau, as for the
+    // MFMA kernels (one run per query and region).
+    if (p.guess && p.path == kPathGeneric) {
+      auto* failed = reinterpret_cast<int*>(ws + p.off_failed);
+      auto* failed_qb = reinterpret_cast<int*>(ws + p.off_failed_qb);
+      knn::SearchArgs<T> sm = sa;
+      sm.kc = knn::kSampleJ;
+      sm.cap = kGenericSampleCap;
+      sm.n_splits = p.sample_regions;
+      sm.rows_per_split = p.sample_rows;
+      sm.split_stride = p.sample_stride;
+      sm.sample_out = reinterpret_cast<float*>(ws + p.off_sample);
+      knn::launch_generic_sample<T>(sm, stream);
+      ma.failed = failed;
+      ma.failed_qblocks = failed_qb;
+      ma.qblock_queries = knn::kGenQ;
+      knn::launch_sample_guess(sm.sample_out, n_test, p.sample_regions, p.sample_j, tau,
+                               reinterpret_cast<float*>(ws + p.off_guess), p.guess_fail_stride,
+                               failed, failed_qb, ma.qblock_queries, stream);
+    }
     if constexpr (sizeof(T) == 4) {
-      if (p.guess) {
+      if (p.guess && p.path == kPathLowd) {
         auto* failed = reinterpret_cast<int*>(ws + p.off_failed);
         auto* failed_qb = reinterpret_cast<int*>(ws + p.off_failed_qb);
         knn::SearchArgs<float> sm = sa;
@@ -688,10 +722,20 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     knn::launch_merge_rerank<T>(m, stream);
   };
   merge(ma);
+  // 5. queries whose sampled guess proved too tight: search again (only
+  // their query blocks, only them), from tau = +inf, and merge them.
+  if (ma.failed && p.path == kPathGeneric) {
+    knn::SearchArgs<T> ra = sa;
+    ra.qblock_active = ma.failed_qblocks;
+    ra.query_active = ma.failed;
+    knn::launch_generic_research<T>(ra, stream);
+    knn::MergeArgs<T> rm = ma;
+    rm.failed = nullptr;
+    rm.only_queries = ma.failed;
+    merge(rm);
+  }
   if constexpr (sizeof(T) == 4) {
-    // 5. queries whose sampled guess proved too tight: search again (only
-    // their query blocks, only them), from tau = +inf, and merge them.
-    if (ma.failed) {
+    if (ma.failed && p.path != kPathGeneric) {
       if (p.path == kPathLowd) {
         knn::SearchArgs<float> ra = sa;
         ra.qblock_active = ma.failed_qblocks;

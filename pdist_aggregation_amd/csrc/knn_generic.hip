@@ -19,6 +19,15 @@
 // keeps the best k' and lowers tau to the k'-th value.  tau is also shared
 // between the workgroups of one query through an atomicMin on an ordered key,
 // so every split filters with the best threshold found anywhere.
+//
+// Two more launches of the same kernel serve the sampled threshold (the
+// guess): the sample launch reads a few regions of rows (split s starts at
+// s * split_stride) and writes each query's kSampleJ smallest reduced
+// distances per region, ascending, rounded up to float, to sample_out without
+// touching tau; the re-search launch runs only the query blocks and queries
+// whose guessed threshold proved too tight (qblock_active / query_active),
+// from tau = +inf.  A plain search starts every query from the global tau,
+// which holds the guess when there is one and +inf otherwise.
 #include "knn_common.hpp"
 #include "knn_kernels.hpp"
 
@@ -26,11 +35,13 @@ namespace knn {
 
 namespace {
 
+constexpr int kGenSearch = 0, kGenSample = 1, kGenResearch = 2;
+
 // Sort and cut to k' every list of the workgroup longer than min_count: each
 // wave takes queries w, w+4, ... and sorts them alone (wave_sort_cut in its
 // own LDS slice sd/si + w * cap), so four lists are sorted at once and no
 // workgroup barrier runs inside a sort.
-template <class T>
+template <class T, bool SHARE>
 __device__ void compact_lists(const SearchArgs<T>& a, T* cd, int* ci, T* tau_s, int* cnt_s,
                               int64_t q0, int min_count, T* sd_all, int* si_all) {
   __syncthreads();  // appends and counts visible
@@ -47,15 +58,16 @@ __device__ void compact_lists(const SearchArgs<T>& a, T* cd, int* ci, T* tau_s, 
       if (m == a.kc) {
         const T kth = sd[a.kc - 1];
         if (kth < tau_s[qq]) tau_s[qq] = kth;
-        atomicMin(&a.tau[q0 + qq], ordered_key((double)kth));
+        if (SHARE) atomicMin(&a.tau[q0 + qq], ordered_key((double)kth));
       }
     }
   }
   __syncthreads();  // counts and thresholds visible to the workgroup
 }
 
-template <class T>
+template <class T, int MODE>
 __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> a) {
+  constexpr bool kShare = MODE != kGenSample;  // a sample's k-th value bounds nothing
   constexpr int kRowPad = 16 / sizeof(T);  // keeps every slice row 16-byte aligned
   __shared__ __align__(16) T xs[kGenK][kGenQ + kRowPad];
   __shared__ __align__(16) T ys[kGenK][kGenY + kRowPad];
@@ -66,10 +78,12 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
   T* sd = reinterpret_cast<T*>(dyn_smem);
   int* si = reinterpret_cast<int*>(dyn_smem + (size_t)(kThreads / 64) * a.cap * sizeof(T));
 
+  if (MODE == kGenResearch && a.qblock_active[blockIdx.x] == 0) return;  // whole workgroup
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
   const int64_t q0 = (int64_t)blockIdx.x * kGenQ;
-  const int64_t ybeg = (int64_t)blockIdx.y * a.rows_per_split;
+  const int64_t ybeg =
+      (int64_t)blockIdx.y * (a.split_stride > 0 ? a.split_stride : a.rows_per_split);
   const int64_t yend = ybeg + a.rows_per_split < a.n_t ? ybeg + a.rows_per_split : a.n_t;
   const size_t blk = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
   T* cd = a.cand_d + blk * kGenQ * a.cap;
@@ -77,7 +91,16 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
   const int64_t n_q_here = a.n_q - q0 < kGenQ ? a.n_q - q0 : kGenQ;
 
   if (tid < kGenQ) {
-    tau_s[tid] = pos_inf<T>();
+    T t = pos_inf<T>();
+    if (kShare && tid < n_q_here) {
+      if (MODE == kGenResearch && a.query_active[q0 + tid] == 0) {
+        t = -pos_inf<T>();  // not re-searched: appends nothing
+      } else {
+        const T g = (T)from_ordered_key(load_agent(&a.tau[q0 + tid]));
+        if (g < t) t = g;
+      }
+    }
+    tau_s[tid] = t;
     cnt_s[tid] = 0;
   }
   if (tid == 0) flag_s = 0;
@@ -137,10 +160,10 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
     }
     __syncthreads();
     if (flag_s) {
-      compact_lists(a, cd, ci, tau_s, cnt_s, q0, a.cap - kGenY, sd, si);
+      compact_lists<T, kShare>(a, cd, ci, tau_s, cnt_s, q0, a.cap - kGenY, sd, si);
       if (tid == 0) flag_s = 0;
     }
-    if ((step & 7) == 7 && tid < n_q_here) {
+    if (kShare && (step & 7) == 7 && tid < n_q_here) {
       const T g = (T)from_ordered_key(load_agent(&a.tau[q0 + tid]));
       if (g < tau_s[tid]) tau_s[tid] = g;
     }
@@ -148,7 +171,20 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
   }
 
   // Final: every list sorted and cut to k'; written to this split's slot.
-  compact_lists(a, cd, ci, tau_s, cnt_s, q0, 0, sd, si);
+  compact_lists<T, kShare>(a, cd, ci, tau_s, cnt_s, q0, 0, sd, si);
+  if (MODE == kGenSample) {
+    // kc == kSampleJ here; rounding up keeps every sampled value >= its row's.
+    for (int e = tid; e < kGenQ * kSampleJ; e += kThreads) {
+      const int qq = e / kSampleJ, j = e % kSampleJ;
+      if (qq >= n_q_here) continue;
+      const int n = cnt_s[qq];
+      float v = __builtin_huge_valf();
+      if (j < n) v = sizeof(T) == 4 ? (float)cd[(size_t)qq * a.cap + j]
+                                    : __double2float_ru((double)cd[(size_t)qq * a.cap + j]);
+      a.sample_out[((size_t)(q0 + qq) * a.n_splits + blockIdx.y) * kSampleJ + j] = v;
+    }
+    return;
+  }
   for (int e = tid; e < kGenQ * a.kc; e += kThreads) {
     const int qq = e / a.kc, j = e % a.kc;
     if (qq >= n_q_here) continue;
@@ -161,14 +197,38 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
 
 }  // namespace
 
-template <class T>
-void launch_generic_search(const SearchArgs<T>& a, hipStream_t s) {
+template <class T, int MODE>
+void launch_generic_mode(const SearchArgs<T>& a, hipStream_t s) {
+  if (a.n_q <= 0) return;
   dim3 grid((unsigned)ceil_div(a.n_q, kGenQ), (unsigned)a.n_splits);
   const size_t lds = (size_t)(kThreads / 64) * a.cap * (sizeof(T) + sizeof(int));
-  hipLaunchKernelGGL(generic_search_kernel<T>, grid, dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL((generic_search_kernel<T, MODE>), grid, dim3(kThreads), lds, s, a);
+}
+
+template <class T>
+void launch_generic_search(const SearchArgs<T>& a, hipStream_t s) {
+  launch_generic_mode<T, kGenSearch>(a, s);
+}
+
+template <class T>
+bool launch_generic_sample(const SearchArgs<T>& a, hipStream_t s) {
+  if (!a.sample_out || a.kc != kSampleJ) return false;
+  launch_generic_mode<T, kGenSample>(a, s);
+  return true;
+}
+
+template <class T>
+bool launch_generic_research(const SearchArgs<T>& a, hipStream_t s) {
+  if (!a.qblock_active || !a.query_active) return false;
+  launch_generic_mode<T, kGenResearch>(a, s);
+  return true;
 }
 
 template void launch_generic_search<float>(const SearchArgs<float>&, hipStream_t);
 template void launch_generic_search<double>(const SearchArgs<double>&, hipStream_t);
+template bool launch_generic_sample<float>(const SearchArgs<float>&, hipStream_t);
+template bool launch_generic_sample<double>(const SearchArgs<double>&, hipStream_t);
+template bool launch_generic_research<float>(const SearchArgs<float>&, hipStream_t);
+template bool launch_generic_research<double>(const SearchArgs<double>&, hipStream_t);
 
 }  // namespace knn

@@ -45,7 +45,7 @@ struct SearchArgs {
   unsigned long long* tau;  // [n_q] ordered key of the best known k'-th value
   T* part_d;                // [n_q][n_splits][kc] sorted candidate lists
   int* part_i;
-  // Low-d kernel only (zero / nullptr: a plain search).  split_stride: rows
+  // Low-d and generic kernels (zero / nullptr: a plain search).  split_stride: rows
   // between the starts of consecutive splits (0: rows_per_split).
   // sample_out: the sampling launch's per-query runs, [n_q][n_splits][kSampleJ]
   // smallest 32-row minima, ascending.  qblock_active / query_active: the
@@ -163,6 +163,13 @@ void launch_row_norms(const T* Y, int64_t n, int d, T* out, DeviceScalars* sc, h
 
 template <class T>
 void launch_generic_search(const SearchArgs<T>& a, hipStream_t s);
+// Sampling launch (kc == kSampleJ; rows [s * split_stride, + rows_per_split)
+// of each of the n_splits regions, writing sample_out) and re-search launch
+// (qblock_active / query_active, 64 queries per block) of the generic kernel.
+template <class T>
+bool launch_generic_sample(const SearchArgs<T>& a, hipStream_t s);
+template <class T>
+bool launch_generic_research(const SearchArgs<T>& a, hipStream_t s);
 
 // Low-dimensional (d < 16) VALU kernel (knn_lowd.hip): 256 queries per
 // workgroup, 32 rows per step.  Returns false for d >= 16.

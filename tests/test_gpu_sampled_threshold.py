@@ -1,5 +1,5 @@
-"""Sampled-threshold start and re-search of the MFMA and low-d paths (needs a
-GPU).
+"""Sampled-threshold start and re-search of the MFMA, low-d and generic paths
+(needs a GPU).
 
 The search first sweeps 1/32 of the train rows, keeps per-lane minima
 and starts every query from a guessed threshold (the j-th smallest sampled
@@ -16,7 +16,7 @@ import pytest
 
 import oracle
 import pdist_aggregation_amd as pa
-from knn_checks import EPS_F32, assert_knn_parity, assert_sorted_unique
+from knn_checks import EPS_F32, EPS_F64, assert_knn_parity, assert_sorted_unique
 
 pytestmark = pytest.mark.gpu
 
@@ -125,3 +125,46 @@ def test_guess_default_off_below_size_and_repeatable(require_gpu, monkeypatch):
     np.testing.assert_array_equal(i1, i2)
     np.testing.assert_array_equal(d1, d2)
     np.testing.assert_array_equal(i0, i1)
+
+
+# Generic kernel (float32 with d not a multiple of 4 or d > 256, any float64):
+# its sampling launch keeps the kSampleJ smallest reduced distances of each
+# region (float64 ones rounded up to float) and its re-search runs 64-query
+# blocks.
+GENERIC_CASES = [(np.float32, 130, 10), (np.float32, 300, 32), (np.float64, 50, 10),
+                 (np.float64, 100, 100)]
+
+
+@pytest.mark.parametrize("mode", ["1", "fail", "fail:7"])
+@pytest.mark.parametrize("dtype,d,k", GENERIC_CASES)
+def test_generic_guess_matches_oracle(require_gpu, monkeypatch, mode, dtype, d, k):
+    monkeypatch.setenv("PDIST_KNN_GUESS", mode)
+    rs = np.random.RandomState(d * 13 + k)
+    Y = rs.rand(30000, d).astype(dtype)
+    X = rs.rand(300, d).astype(dtype)
+    _say(f"[generic guess {mode}] {np.dtype(dtype).name} d={d} k={k} 30000 x 300")
+    dist, idx, st = _run(Y, X, k, "exact")
+    _say(f"    stats {st}")
+    assert st["kernel_path"] == 0
+    if mode == "fail":
+        assert st["n_rescued"] == X.shape[0]
+    elif mode == "fail:7":
+        assert st["n_rescued"] == len(range(0, X.shape[0], 7))
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, k, EPS_F64 if dtype == np.float64 else EPS_F32)
+
+
+def test_generic_guess_on_by_default_and_repeatable(require_gpu, monkeypatch):
+    """Float64 with 40,000 train rows takes the sample by default; the answer
+    equals the one searched from +inf (sampling forced off)."""
+    monkeypatch.delenv("PDIST_KNN_GUESS", raising=False)
+    rs = np.random.RandomState(21)
+    Y = rs.rand(40000, 50)
+    X = rs.rand(500, 50)
+    d1, i1, st1 = _run(Y, X, 10, "exact")
+    monkeypatch.setenv("PDIST_KNN_GUESS", "0")
+    d0, i0, _ = _run(Y, X, 10, "exact")
+    _say(f"    float64 default: stats {st1}")
+    np.testing.assert_array_equal(i0, i1)
+    np.testing.assert_array_equal(d0, d1)
+    assert_knn_parity(X, Y, d1, i1, 10, EPS_F64)
