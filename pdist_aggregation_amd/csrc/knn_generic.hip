@@ -65,6 +65,105 @@ __device__ void compact_lists(const SearchArgs<T>& a, T* cd, int* ci, T* tau_s, 
   __syncthreads();  // counts and thresholds visible to the workgroup
 }
 
+// compact_lists for the sampling launch (no tau sharing).  Wave w takes the
+// queries 4w..4w+3, 4w+16.., ... four at a time.  When none of the four lists
+// holds more than 32 entries (every list of a region of up to 32 rows), each
+// 16-lane quarter of the wave sorts one of them in its own 32-entry slice of
+// the wave's LDS scratch (the capacity is 128), so the four sorts run
+// together; longer lists are sorted one after another by the whole wave
+// (wave_sort_cut).  Both are the same total order on (value, index), so the
+// kept entries do not depend on which form ran.  Used by the sampling launch
+// only: in the search launches the extra code costs the step loop registers
+// and instruction-cache room (float64 1e5 x 1e3 x 500 search 4.0 -> 4.5 ms).
+// The loops stay rolled for the same reason (sampling launch at 1e4 x 1e3 x
+// 50 float64: 0.10 ms unrolled, 0.04 ms rolled).
+template <class T>
+__device__ void compact_lists_sample(const SearchArgs<T>& a, T* cd, int* ci, T* tau_s, int* cnt_s,
+                                     int min_count, T* sd_all, int* si_all) {
+  constexpr int kSmall = 32;  // entries per list a 16-lane quarter sorts
+  __syncthreads();  // appends and counts visible
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  T* sd = sd_all + (size_t)w * a.cap;
+  int* si = si_all + (size_t)w * a.cap;
+#pragma unroll 1
+  for (int base = 4 * w; base < kGenQ; base += 4 * (kThreads / 64)) {
+    int nmax = 0;  // uniform: LDS values written before the barrier or by this wave
+#pragma unroll
+    for (int g = 0; g < 4; ++g) nmax = max(nmax, cnt_s[base + g]);
+    if (nmax <= min_count) continue;
+    if (nmax <= kSmall) {
+      const int g = lane >> 4, gl = lane & 15;
+      const int qq = base + g;
+      const int n = cnt_s[qq];
+      const bool act = n > min_count;
+      const int P = next_pow2(nmax);
+      T* gd = sd + g * kSmall;
+      int* gi = si + g * kSmall;
+      T* cq = cd + (size_t)qq * a.cap;
+      int* iq = ci + (size_t)qq * a.cap;
+      for (int e = gl; e < P; e += 16) {
+        const bool ok = e < n;
+        gd[e] = ok ? cq[e] : pos_inf<T>();
+        gi[e] = ok ? iq[e] : INT_MAX;
+      }
+      wave_sync();
+#pragma unroll 1
+      for (int size = 2; size <= P; size <<= 1) {
+#pragma unroll 1
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+          if (gl < (P >> 1)) {
+            const int lo = ((gl & ~(stride - 1)) << 1) | (gl & (stride - 1));
+            const int hi = lo + stride;
+            const bool up = (lo & size) == 0;
+            const T dl = gd[lo], dh = gd[hi];
+            const int il = gi[lo], ih = gi[hi];
+            const bool swap = up ? key_less(dh, ih, dl, il) : key_less(dl, il, dh, ih);
+            if (swap) {
+              gd[lo] = dh;
+              gd[hi] = dl;
+              gi[lo] = ih;
+              gi[hi] = il;
+            }
+          }
+          wave_sync();
+        }
+      }
+      const int m = n < a.kc ? n : a.kc;
+      if (act) {
+        for (int e = gl; e < m; e += 16) {
+          cq[e] = gd[e];
+          iq[e] = gi[e];
+        }
+        if (gl == 0) {
+          cnt_s[qq] = m;
+          if (m == a.kc) {
+            const T kth = gd[a.kc - 1];
+            if (kth < tau_s[qq]) tau_s[qq] = kth;
+          }
+        }
+      }
+      wave_sync();
+      continue;
+    }
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+      const int qq = base + g;
+      const int n = cnt_s[qq];
+      if (n <= min_count) continue;
+      wave_sort_cut(cd + (size_t)qq * a.cap, ci + (size_t)qq * a.cap, n, a.kc, sd, si, lane);
+      if (lane == 0) {
+        const int m = n < a.kc ? n : a.kc;
+        cnt_s[qq] = m;
+        if (m == a.kc) {
+          const T kth = sd[a.kc - 1];
+          if (kth < tau_s[qq]) tau_s[qq] = kth;
+        }
+      }
+    }
+  }
+  __syncthreads();  // counts and thresholds visible to the workgroup
+}
+
 template <class T, int MODE>
 __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> a) {
   constexpr bool kShare = MODE != kGenSample;  // a sample's k-th value bounds nothing
@@ -160,7 +259,10 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
     }
     __syncthreads();
     if (flag_s) {
-      compact_lists<T, kShare>(a, cd, ci, tau_s, cnt_s, q0, a.cap - kGenY, sd, si);
+      if constexpr (MODE == kGenSample)
+        compact_lists_sample<T>(a, cd, ci, tau_s, cnt_s, a.cap - kGenY, sd, si);
+      else
+        compact_lists<T, kShare>(a, cd, ci, tau_s, cnt_s, q0, a.cap - kGenY, sd, si);
       if (tid == 0) flag_s = 0;
     }
     if (kShare && (step & 7) == 7 && tid < n_q_here) {
@@ -171,7 +273,10 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
   }
 
   // Final: every list sorted and cut to k'; written to this split's slot.
-  compact_lists<T, kShare>(a, cd, ci, tau_s, cnt_s, q0, 0, sd, si);
+  if constexpr (MODE == kGenSample)
+    compact_lists_sample<T>(a, cd, ci, tau_s, cnt_s, 0, sd, si);
+  else
+    compact_lists<T, kShare>(a, cd, ci, tau_s, cnt_s, q0, 0, sd, si);
   if (MODE == kGenSample) {
     // kc == kSampleJ here; rounding up keeps every sampled value >= its row's.
     for (int e = tid; e < kGenQ * kSampleJ; e += kThreads) {
