@@ -87,6 +87,23 @@ def _as_device(a, name, torch, device):
     return torch.from_numpy(arr).to(device), True
 
 
+def _to_host(tensors, torch, stream):
+    """numpy copies of device tensors, through pinned host buffers.
+
+    The copies are queued on ``stream`` (after the search) into page-locked
+    buffers from torch's caching host allocator, which are reused from call to
+    call; pageable copies of the results ran at a few GB/s (1e5 x 100 float64
+    distances + int64 indices: about 40 ms).  The arrays keep their buffers
+    alive.
+    """
+    hosts = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True) for t in tensors]
+    with torch.cuda.stream(stream):
+        for h, t in zip(hosts, tensors):
+            h.copy_(t, non_blocking=True)
+    stream.synchronize()
+    return [h.numpy() for h in hosts]
+
+
 def parallel_knn(X_train, X_test, k, chunk_size=CHUNK_SIZE, strategy="auto",
                  return_distance=False, *, squared=False, precision="exact",
                  oversample=None, stream=None, return_stats=False, n_threads=None):
@@ -155,7 +172,10 @@ def parallel_knn(X_train, X_test, k, chunk_size=CHUNK_SIZE, strategy="auto",
     _capi.check(code)
     n_chunks = n_parallel_chunks(strategy, n_train, n_test, chunk_size, threads)
     if not (isinstance(X_test, torch.Tensor) and X_test.is_cuda):
-        dist_o, idx_o = dist.cpu().numpy(), idx.cpu().numpy()
+        if return_distance:
+            dist_o, idx_o = _to_host([dist, idx], torch, s)
+        else:
+            dist_o, (idx_o,) = None, _to_host([idx], torch, s)
     else:
         dist_o, idx_o = dist, idx
     out = ((dist_o, idx_o), n_chunks) if return_distance else (idx_o, n_chunks)
