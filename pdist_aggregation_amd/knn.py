@@ -82,6 +82,7 @@ def _upload(src, torch, device, stream):
     buffer: about 50 GB/s on the MI355X's PCIe link against about 28 GB/s for
     a pageable copy (400 MiB float64: 14.7 -> 8.1 ms).
     """
+    src = src.detach()  # the search is not differentiated through this path
     if src.numel() * src.element_size() < _UPLOAD_MIN_BYTES:
         with torch.cuda.stream(stream):
             return src.to(device)
