@@ -359,7 +359,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   }
   const bool mfma = p.path == kPathMfmaF32 || p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8;
   if (mfma) plan_guess(p, slots, 2);
-  if (p.path == kPathLowd && elem == 4) plan_guess(p, slots, 1);
+  if (p.path == kPathLowd) plan_guess(p, slots, 1);
   // Generic kernel: one run per (query, region), from as many sampling
   // workgroups as the search's own grid target.  Only with 4 or more splits:
   // the guess saves each split's first fill and compaction, and with fewer
@@ -530,7 +530,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     sa.part_d = reinterpret_cast<T*>(ws + p.off_part_d);
     sa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
     // 2b. sampled threshold: the kSampleJ smallest over 1/32 of the rows
-    // (generic; low-d float: of 32-row minima) -> guess -> tau, as for the
+    // (generic; low-d: of 32-row minima; float64 values rounded up) -> guess -> tau, as for the
     // MFMA kernels (one run per query and region).
     if (p.guess && p.path == kPathGeneric) {
       auto* failed = reinterpret_cast<int*>(ws + p.off_failed);
@@ -550,23 +550,21 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
                                reinterpret_cast<float*>(ws + p.off_guess), p.guess_fail_stride,
                                failed, failed_qb, ma.qblock_queries, stream);
     }
-    if constexpr (sizeof(T) == 4) {
-      if (p.guess && p.path == kPathLowd) {
-        auto* failed = reinterpret_cast<int*>(ws + p.off_failed);
-        auto* failed_qb = reinterpret_cast<int*>(ws + p.off_failed_qb);
-        knn::SearchArgs<float> sm = sa;
-        sm.n_splits = p.sample_regions;
-        sm.rows_per_split = p.sample_rows;
-        sm.split_stride = p.sample_stride;
-        sm.sample_out = reinterpret_cast<float*>(ws + p.off_sample);
-        knn::launch_lowd_sample(sm, stream);
-        ma.failed = failed;
-        ma.failed_qblocks = failed_qb;
-        ma.qblock_queries = knn::kThreads;
-        knn::launch_sample_guess(sm.sample_out, n_test, p.sample_regions, p.sample_j, tau,
-                                 reinterpret_cast<float*>(ws + p.off_guess), p.guess_fail_stride,
-                                 failed, failed_qb, ma.qblock_queries, stream);
-      }
+    if (p.guess && p.path == kPathLowd) {
+      auto* failed = reinterpret_cast<int*>(ws + p.off_failed);
+      auto* failed_qb = reinterpret_cast<int*>(ws + p.off_failed_qb);
+      knn::SearchArgs<T> sm = sa;
+      sm.n_splits = p.sample_regions;
+      sm.rows_per_split = p.sample_rows;
+      sm.split_stride = p.sample_stride;
+      sm.sample_out = reinterpret_cast<float*>(ws + p.off_sample);
+      knn::launch_lowd_sample<T>(sm, stream);
+      ma.failed = failed;
+      ma.failed_qblocks = failed_qb;
+      ma.qblock_queries = knn::kThreads;
+      knn::launch_sample_guess(sm.sample_out, n_test, p.sample_regions, p.sample_j, tau,
+                               reinterpret_cast<float*>(ws + p.off_guess), p.guess_fail_stride,
+                               failed, failed_qb, ma.qblock_queries, stream);
     }
     timer.mark(1);
     if (p.path == kPathLowd)
@@ -724,29 +722,25 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   merge(ma);
   // 5. queries whose sampled guess proved too tight: search again (only
   // their query blocks, only them), from tau = +inf, and merge them.
-  if (ma.failed && p.path == kPathGeneric) {
+  if (ma.failed && (p.path == kPathGeneric || p.path == kPathLowd)) {
     knn::SearchArgs<T> ra = sa;
     ra.qblock_active = ma.failed_qblocks;
     ra.query_active = ma.failed;
-    knn::launch_generic_research<T>(ra, stream);
+    if (p.path == kPathLowd)
+      knn::launch_lowd_research<T>(ra, stream);
+    else
+      knn::launch_generic_research<T>(ra, stream);
     knn::MergeArgs<T> rm = ma;
     rm.failed = nullptr;
     rm.only_queries = ma.failed;
     merge(rm);
   }
   if constexpr (sizeof(T) == 4) {
-    if (ma.failed && p.path != kPathGeneric) {
-      if (p.path == kPathLowd) {
-        knn::SearchArgs<float> ra = sa;
-        ra.qblock_active = ma.failed_qblocks;
-        ra.query_active = ma.failed;
-        knn::launch_lowd_research(ra, stream);
-      } else {
-        knn::MfmaArgs ra = fa;
-        ra.qblock_active = ma.failed_qblocks;
-        ra.query_active = ma.failed;
-        knn::launch_mfma_research(ra, stream);
-      }
+    if (ma.failed && p.path != kPathGeneric && p.path != kPathLowd) {
+      knn::MfmaArgs ra = fa;
+      ra.qblock_active = ma.failed_qblocks;
+      ra.query_active = ma.failed;
+      knn::launch_mfma_research(ra, stream);
       knn::MergeArgs<T> rm = ma;
       rm.failed = nullptr;
       rm.only_queries = ma.failed;

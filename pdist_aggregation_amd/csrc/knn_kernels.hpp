@@ -180,10 +180,12 @@ using LowdArgs = SearchArgs<T>;
 template <class T>
 bool launch_lowd_search(const LowdArgs<T>& a, hipStream_t s);
 // Sampling launch (rows [s * split_stride, + rows_per_split) of each of the
-// n_splits regions, writing sample_out) and re-search launch (qblock_active /
-// query_active) of the same kernel; float only.
-bool launch_lowd_sample(const LowdArgs<float>& a, hipStream_t s);
-bool launch_lowd_research(const LowdArgs<float>& a, hipStream_t s);
+// n_splits regions, writing sample_out as float, double minima rounded up)
+// and re-search launch (qblock_active / query_active) of the same kernel.
+template <class T>
+bool launch_lowd_sample(const LowdArgs<T>& a, hipStream_t s);
+template <class T>
+bool launch_lowd_research(const LowdArgs<T>& a, hipStream_t s);
 template <class T>
 int lowd_blocks_per_cu(int d, int cap);
 

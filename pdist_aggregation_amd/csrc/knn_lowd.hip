@@ -129,7 +129,10 @@ __global__ __launch_bounds__(kThreads) void lowd_search_kernel(LowdArgs<T> a) {
     for (int r = 0; r + 1 < kRows; r += 2) m = fmin(m, fmin(v[r], v[r + 1]));
     if constexpr (MODE == kLowdSample) {
       // Insert m into the sorted run: b[i] <- min(b[i], max(b[i-1], m)).
-      const float mf = (float)m;  // a NaN drops this step's minimum
+      // Rounded up for double (a sampled value never below its row's); a NaN
+      // becomes +inf, which leaves the run unchanged.
+      const float m32 = sizeof(T) == 4 ? (float)m : __double2float_ru((double)m);
+      const float mf = m32 == m32 ? m32 : __builtin_huge_valf();
       if (__any(qok && mf < best[kSampleJ - 1])) {
 #pragma unroll
         for (int i = kSampleJ - 1; i > 0; --i) best[i] = fminf(best[i], fmaxf(best[i - 1], mf));
@@ -257,12 +260,14 @@ bool launch_lowd_search(const LowdArgs<T>& a, hipStream_t s) {
   return launch_mode<T, kLowdSearch>(a, s);
 }
 
-bool launch_lowd_sample(const LowdArgs<float>& a, hipStream_t s) {
-  return a.sample_out && launch_mode<float, kLowdSample>(a, s);
+template <class T>
+bool launch_lowd_sample(const LowdArgs<T>& a, hipStream_t s) {
+  return a.sample_out && launch_mode<T, kLowdSample>(a, s);
 }
 
-bool launch_lowd_research(const LowdArgs<float>& a, hipStream_t s) {
-  return a.qblock_active && a.query_active && launch_mode<float, kLowdResearch>(a, s);
+template <class T>
+bool launch_lowd_research(const LowdArgs<T>& a, hipStream_t s) {
+  return a.qblock_active && a.query_active && launch_mode<T, kLowdResearch>(a, s);
 }
 
 template <class T>
@@ -276,5 +281,9 @@ template int lowd_blocks_per_cu<float>(int, int);
 template int lowd_blocks_per_cu<double>(int, int);
 template bool launch_lowd_search<float>(const LowdArgs<float>&, hipStream_t);
 template bool launch_lowd_search<double>(const LowdArgs<double>&, hipStream_t);
+template bool launch_lowd_sample<float>(const LowdArgs<float>&, hipStream_t);
+template bool launch_lowd_sample<double>(const LowdArgs<double>&, hipStream_t);
+template bool launch_lowd_research<float>(const LowdArgs<float>&, hipStream_t);
+template bool launch_lowd_research<double>(const LowdArgs<double>&, hipStream_t);
 
 }  // namespace knn

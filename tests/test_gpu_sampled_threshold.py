@@ -168,3 +168,24 @@ def test_generic_guess_on_by_default_and_repeatable(require_gpu, monkeypatch):
     np.testing.assert_array_equal(i0, i1)
     np.testing.assert_array_equal(d0, d1)
     assert_knn_parity(X, Y, d1, i1, 10, EPS_F64)
+
+
+@pytest.mark.parametrize("mode", ["1", "fail", "fail:7"])
+@pytest.mark.parametrize("d,k", [(3, 10), (7, 16), (12, 5)])
+def test_lowd_float64_guess_matches_oracle(require_gpu, monkeypatch, mode, d, k):
+    """Low-d kernel on float64: sampled minima rounded up to float, re-search
+    of 256-query blocks."""
+    monkeypatch.setenv("PDIST_KNN_GUESS", mode)
+    rs = np.random.RandomState(d * 17 + k)
+    Y = rs.rand(30000, d)
+    X = rs.rand(600, d)
+    _say(f"[low-d float64 guess {mode}] d={d} k={k} 30000 x 600")
+    dist, idx, st = _run(Y, X, k, "exact")
+    _say(f"    stats {st}")
+    assert st["kernel_path"] == 4
+    if mode == "fail":
+        assert st["n_rescued"] == X.shape[0]
+    elif mode == "fail:7":
+        assert st["n_rescued"] == len(range(0, X.shape[0], 7))
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, k, EPS_F64)
