@@ -71,6 +71,7 @@ def _torch():
 
 _UPLOAD_MIN_BYTES = 4 << 20  # host arrays from this size go through pinned staging
 _UPLOAD_CHUNK_BYTES = 8 << 20
+_PINNED_RESULT_MAX_BYTES = 1 << 30  # dense matrices above this come back pageable
 
 
 def _upload(src, torch, device, stream):
@@ -250,5 +251,8 @@ def sqdist_matrix(X_train, X_test, stream=None):
                   out.data_ptr(), ctypes.c_void_p(s.cuda_stream))
     _capi.check(code)
     if not (isinstance(X_test, torch.Tensor) and X_test.is_cuda):
-        return out.cpu().numpy()
+        if out.numel() * out.element_size() <= _PINNED_RESULT_MAX_BYTES:
+            return _to_host([out], torch, s)[0]
+        with torch.cuda.stream(s):  # larger: no pinned block of this size kept around
+            return out.cpu().numpy()
     return out
