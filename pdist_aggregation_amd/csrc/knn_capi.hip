@@ -69,6 +69,7 @@ struct StageTimer {
 struct Plan {
   int64_t n_train = 0, n_test = 0, dim = 0, k = 0;
   int kc = 0, cap = 0, n_splits = 1, path = 0, dm = 0;
+  int dpad = 0;  // float32 MFMA, d % 4 != 0: row width of the zero-padded copies
   // Two-level merge: with more than kMaxMerge / k' splits, groups of
   // group_size split lists are first reduced to n_groups lists per query.
   int group_size = 1, n_groups = 1;
@@ -87,7 +88,8 @@ struct Plan {
   size_t off_scalars = 0, off_ynorm = 0, off_tau = 0, off_cand_d = 0, off_cand_i = 0,
          off_part_d = 0, off_part_i = 0, off_mu = 0, off_xq = 0, off_yq = 0, off_xnorm = 0,
          off_xerr = 0, off_xscale = 0, off_yscale = 0, off_slots = 0, off_sample = 0,
-         off_guess = 0, off_failed = 0, off_failed_qb = 0, off_uncert = 0, total = 0;
+         off_guess = 0, off_failed = 0, off_failed_qb = 0, off_uncert = 0, off_xpad = 0,
+         off_ypad = 0, total = 0;
 };
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
@@ -291,12 +293,15 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
                                         std::to_string(knn::kMfmaMaxCap - knn::kMfmaCapSlack));
     p.path = precision == KNN_PRECISION_BF16 ? kPathMfmaBF16 : kPathMfmaFP8;
     p.dm = std::max(p.path == kPathMfmaBF16 ? 32 : 64, knn::next_pow2((int)dim));
-  } else if (!force_generic && elem == 4 && dim % 4 == 0 && dim >= 16 && dim <= 256 &&
+  } else if (!force_generic && elem == 4 && dim >= 16 && dim <= 256 &&
              mfma_cap_ok) {
     p.path = kPathMfmaF32;
     // 192 is a search width too (48 chunks of 16 bytes per row): d in
     // (128, 192] would otherwise pad to 256 and spend up to half the MFMAs on zeros.
     p.dm = dim <= 128 ? std::max(32, knn::next_pow2((int)dim)) : dim <= 192 ? 192 : 256;
+    // d % 4 != 0: the search reads zero-padded copies with 16-byte aligned
+    // rows (the exact re-rank still reads the caller's rows).
+    if (dim % 4 != 0) p.dpad = (int)((dim + 3) & ~(int64_t)3);
   } else if (dim < 16 && lowd_cap <= knn::kMfmaMaxCap) {
     p.path = kPathLowd;
   }
@@ -340,7 +345,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     splits = choose_splits_slots(p.n_qblocks, n_train, step, p.kc, slots);
     // The MFMA kernel addresses one split with 32-bit buffer offsets.
     if (p.path != kPathLowd) {
-      const int64_t row_bytes = p.path == kPathMfmaF32 ? dim * 4
+      const int64_t row_bytes = p.path == kPathMfmaF32 ? (p.dpad ? p.dpad : dim) * 4
                                 : p.path == kPathMfmaBF16 ? (int64_t)p.dm * 2
                                                           : (int64_t)p.dm;
       const int64_t min_splits = knn::ceil_div(n_train * row_bytes, (int64_t)1 << 30);
@@ -399,6 +404,10 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     p.off_failed = take((size_t)nq1 * sizeof(int));
     p.off_failed_qb = take((size_t)p.n_qblocks * sizeof(int));
   }
+  if (p.dpad) {
+    p.off_xpad = take((size_t)nq1 * p.dpad * sizeof(float));
+    p.off_ypad = take((size_t)n_train * p.dpad * sizeof(float));
+  }
   if (p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8) {
     const size_t qe = p.path == kPathMfmaBF16 ? 2 : 1;
     p.off_mu = take((1 + (size_t)knn::column_mean_blocks(std::min<int64_t>(n_train, 65536))) *
@@ -445,7 +454,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   Plan p;
   int rc = make_plan(p, sizeof(T), n_train, n_test, dim, k, precision, oversample);
   if (rc != KNN_OK) return rc;
-  if (p.path == kPathMfmaF32 && ((reinterpret_cast<uintptr_t>(X_train) |
+  if (p.path == kPathMfmaF32 && !p.dpad && ((reinterpret_cast<uintptr_t>(X_train) |
                                   reinterpret_cast<uintptr_t>(X_test)) & 15) != 0) {
     rc = make_plan(p, sizeof(T), n_train, n_test, dim, k, precision, oversample, true);
     if (rc != KNN_OK) return rc;
@@ -612,10 +621,23 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       knn::launch_row_norms<float>(reinterpret_cast<const float*>(X_train), n_train, (int)dim,
                                    reinterpret_cast<float*>(ynorm), sc, stream);
       fa.fmt = knn::kFmtF32;
-      fa.X = X_test;
-      fa.Y = X_train;
-      fa.ldx = dim;
-      fa.ldy = dim;
+      if (p.dpad) {
+        float* xp = reinterpret_cast<float*>(ws + p.off_xpad);
+        float* yp = reinterpret_cast<float*>(ws + p.off_ypad);
+        knn::launch_pad_rows(reinterpret_cast<const float*>(X_train), n_train, (int)dim, p.dpad,
+                             yp, stream);
+        knn::launch_pad_rows(reinterpret_cast<const float*>(X_test), n_test, (int)dim, p.dpad, xp,
+                             stream);
+        fa.X = xp;
+        fa.Y = yp;
+        fa.ldx = p.dpad;
+        fa.ldy = p.dpad;
+      } else {
+        fa.X = X_test;
+        fa.Y = X_train;
+        fa.ldx = dim;
+        fa.ldy = dim;
+      }
       fa.d = (int)dim;
     } else {
       // 2. centre + bf16 rounding of both sets, norms and error bounds

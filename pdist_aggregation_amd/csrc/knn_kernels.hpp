@@ -155,6 +155,9 @@ void launch_quantize_fp8(const float* src, int64_t n, int d, int dm, const float
                          unsigned char* scale, float* norm, float* err, DeviceScalars* sc,
                          bool track_max, hipStream_t s);
 
+// Rows of d floats copied to rows of dp floats (dp >= d), zero-filled.
+void launch_pad_rows(const float* src, int64_t n, int d, int dp, float* dst, hipStream_t s);
+
 template <class T>
 void launch_fill_keys(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 

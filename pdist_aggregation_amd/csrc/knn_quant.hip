@@ -450,6 +450,32 @@ void launch_quantize_fp8(const float* src, int64_t n, int d, int dm, const float
                      track_max ? 1 : 0);
 }
 
+namespace {
+
+// n rows of d floats copied into rows of dp >= d floats, zero-filled: the
+// float32 MFMA search then reads 16-byte aligned rows when d % 4 != 0.
+__global__ __launch_bounds__(kThreads) void pad_rows_kernel(const float* __restrict__ src,
+                                                            int64_t n, int d, int dp,
+                                                            float* __restrict__ dst) {
+  const int64_t total = n * dp;
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * kThreads) {
+    const int64_t r = e / dp;
+    const int c = (int)(e - r * dp);
+    dst[e] = c < d ? src[r * d + c] : 0.f;
+  }
+}
+
+}  // namespace
+
+void launch_pad_rows(const float* src, int64_t n, int d, int dp, float* dst, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t want = ceil_div(n * dp, (int64_t)kThreads);
+  const int64_t blocks = want < 8192 ? want : 8192;
+  hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, src, n, d, dp,
+                     dst);
+}
+
 void launch_column_mean(const float* Y, int64_t n_sample, int d, float* part, float* mu,
                         hipStream_t s) {
   const int64_t blocks = column_mean_blocks(n_sample);

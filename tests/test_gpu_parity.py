@@ -229,9 +229,12 @@ def test_full_scale_f32_subsample_and_properties(require_gpu):
 
 @pytest.mark.parametrize("d,expect_path", [(16, 1), (32, 1), (48, 1), (64, 1), (100, 1),
                                            (128, 1), (132, 1), (192, 1), (200, 1), (256, 1),
-                                           (50, 0), (130, 0), (260, 0)])
+                                           (50, 1), (130, 1), (17, 1), (255, 1), (15, 4),
+                                           (260, 0), (301, 0)])
 def test_f32_kernel_path_selection(require_gpu, d, expect_path):
-    """Aligned float32 shapes with 16 <= d <= 256 (d % 4 == 0) run on the MFMA kernel."""
+    """Float32 with 16 <= d <= 256 runs on the MFMA kernel (d % 4 != 0 through
+    zero-padded copies of the rows), d < 16 on the low-d kernel, d > 256 on the
+    generic one."""
     rs = np.random.RandomState(d)
     Y = rs.rand(2111, d).astype(np.float32)
     X = rs.rand(301, d).astype(np.float32)

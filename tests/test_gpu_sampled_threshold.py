@@ -127,11 +127,11 @@ def test_guess_default_off_below_size_and_repeatable(require_gpu, monkeypatch):
     np.testing.assert_array_equal(i0, i1)
 
 
-# Generic kernel (float32 with d not a multiple of 4 or d > 256, any float64):
+# Generic kernel (float32 with d > 256, float64 with d >= 16):
 # its sampling launch keeps the kSampleJ smallest reduced distances of each
 # region (float64 ones rounded up to float) and its re-search runs 64-query
 # blocks.
-GENERIC_CASES = [(np.float32, 130, 10), (np.float32, 300, 32), (np.float64, 50, 10),
+GENERIC_CASES = [(np.float32, 261, 10), (np.float32, 300, 32), (np.float64, 50, 10),
                  (np.float64, 100, 100)]
 
 
