@@ -15,6 +15,9 @@
  *    knn_last_error() gives the message of the last failure on this thread.
  *  - Result order: ascending exact distance, the lower train index first on
  *    equal distances.
+ *  - Environment variables PDIST_KNN_GUESS, PDIST_KNN_SPLITS and
+ *    PDIST_KNN_WIDEN are unsupported testing switches: the library reads them
+ *    only when PDIST_KNN_TESTING=1 is also set.
  */
 #ifndef PDIST_KNN_H
 #define PDIST_KNN_H
@@ -61,6 +64,11 @@ typedef struct knn_stats {
   double merge_ms;           /* KNN_FLAG_TIMING: merge + exact re-rank (+ re-search) */
   int64_t n_rescued;         /* queries searched a second time because the threshold
                                 guessed from a row sample was below their k'-th value */
+  int64_t n_widened;         /* queries (MFMA paths) the error bound did not certify from
+                                their k' candidates, re-ranked exactly over every train
+                                row that can lie within their k-th exact distance; such
+                                a query counts in n_uncertified only if that row set
+                                overflowed its capacity */
 } knn_stats;
 
 /*

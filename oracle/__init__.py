@@ -65,11 +65,17 @@ def brute_force_knn(X, Y, k):
 
 
 def _direct_sqdist(X, Y, idx):
-    """sum((X[i] - Y[idx[i, j]])^2) in FP64 for every (i, j)."""
+    """sum((X[i] - Y[idx[i, j]])^2) in FP64 for every (i, j), in blocks of
+    queries of at most about 128 MiB of gathered rows."""
+    X = np.asarray(X, np.float64)
+    Y = np.asarray(Y, np.float64)
+    idx = np.asarray(idx)
     out = np.empty(idx.shape, np.float64)
-    for r in range(idx.shape[0]):
-        diff = Y[idx[r]] - X[r][None, :]
-        out[r] = np.einsum("ij,ij->i", diff, diff)
+    per = max(1, idx.shape[1] * X.shape[1])
+    b = max(1, (1 << 24) // per)
+    for r0 in range(0, idx.shape[0], b):
+        diff = Y[idx[r0:r0 + b]] - X[r0:r0 + b, None, :]
+        out[r0:r0 + b] = np.einsum("ijk,ijk->ij", diff, diff)
     return out
 
 
@@ -80,7 +86,8 @@ def exact_knn_f64(X, Y, k, extra=None, y_chunk=1 << 16, return_boundary=False):
     ``|y|^2 - 2 x.y`` per query (error ~1e-13 relative), then every candidate
     is re-ranked with the direct-difference FP64 distance and the rows are
     ordered by (distance, index).  ``extra`` (default max(16, k)) absorbs
-    exact ties between duplicate rows and expansion rounding.
+    exact ties between duplicate rows and expansion rounding.  Queries are
+    processed in blocks so that memory stays bounded at any size.
 
     With ``return_boundary`` also returns the (k+1)-th exact distance of each
     row (``inf`` when Y has only k rows), used for near-tie analysis.
@@ -88,16 +95,36 @@ def exact_knn_f64(X, Y, k, extra=None, y_chunk=1 << 16, return_boundary=False):
     X = np.ascontiguousarray(X, np.float64)
     Y = np.ascontiguousarray(Y, np.float64)
     nq, nt = X.shape[0], Y.shape[0]
+    d = X.shape[1]
     kk = min(nt, k + (max(16, k) if extra is None else extra))
+    m = min(nt, k + 1)
     ynorm = np.einsum("ij,ij->i", Y, Y)
-    xnorm = np.einsum("ij,ij->i", X, X)
+    dist = np.empty((nq, m))
+    idx = np.empty((nq, m), np.int64)
+    # query block: the value matrix (block x y_chunk) and the gathered
+    # candidate rows (block x ~kk x d) each stay within about 256 MiB
+    qb = max(1, min((1 << 25) // min(nt, y_chunk), (1 << 25) // max(1, 2 * kk * d)))
+    for q0 in range(0, nq, qb):
+        Xb = X[q0:q0 + qb]
+        dist[q0:q0 + qb], idx[q0:q0 + qb] = _exact_block(Xb, Y, ynorm, kk, m, y_chunk)
+    boundary = dist[:, k].copy() if m > k else np.full(nq, np.inf)
+    dist, idx = dist[:, :k], idx[:, :k]
+    if return_boundary:
+        return dist, idx, boundary
+    return dist, idx
+
+
+def _exact_block(Xb, Y, ynorm, kk, m, y_chunk):
+    """exact_knn_f64 for one block of queries: the m best (distance, index)."""
+    nb, nt = Xb.shape[0], Y.shape[0]
+    xnorm = np.einsum("ij,ij->i", Xb, Xb)
 
     def values(t0):
         Yc = Y[t0:t0 + y_chunk]
-        return ynorm[None, t0:t0 + Yc.shape[0]] - 2.0 * (X @ Yc.T)
+        return ynorm[None, t0:t0 + Yc.shape[0]] - 2.0 * (Xb @ Yc.T)
 
-    # Pass 1: the kk-th smallest expansion value of every row.
-    best = np.full((nq, 0), np.inf)
+    # Pass 1: the kk-th smallest expansion value of every query.
+    best = np.full((nb, 0), np.inf)
     for t0 in range(0, nt, y_chunk):
         v = np.concatenate([best, values(t0)], axis=1)
         best = np.partition(v, kk - 1, axis=1)[:, :kk] if v.shape[1] > kk else v
@@ -105,25 +132,22 @@ def exact_knn_f64(X, Y, k, extra=None, y_chunk=1 << 16, return_boundary=False):
     # Pass 2: every row within rounding of that value is a candidate, so exact
     # ties at the cut (duplicate rows) are all kept, whatever their number.
     margin = 1e-9 * (1.0 + np.abs(thr) + xnorm + ynorm.max())
-    cand = [[] for _ in range(nq)]
+    rows, cols = [], []
     for t0 in range(0, nt, y_chunk):
-        rows, cols = np.nonzero(values(t0) <= (thr + margin)[:, None])
-        for r in np.unique(rows):
-            cand[r].append(cols[rows == r] + t0)
-    m = min(nt, k + 1)
-    dist = np.empty((nq, m))
-    idx = np.empty((nq, m), np.int64)
-    for r in range(nq):
-        ci = np.concatenate(cand[r])
-        diff = Y[ci] - X[r][None, :]
-        cd = np.einsum("ij,ij->i", diff, diff)
-        o = np.lexsort((ci, cd))[:m]
-        dist[r], idx[r] = cd[o], ci[o]
-    boundary = dist[:, k].copy() if m > k else np.full(nq, np.inf)
-    dist, idx = dist[:, :k], idx[:, :k]
-    if return_boundary:
-        return dist, idx, boundary
-    return dist, idx
+        r, c = np.nonzero(values(t0) <= (thr + margin)[:, None])
+        rows.append(r)
+        cols.append(c + t0)
+    rows = np.concatenate(rows)
+    cols = np.concatenate(cols)
+    diff = Y[cols] - Xb[rows]
+    cd = np.einsum("ij,ij->i", diff, diff)
+    # order by (query, exact distance, index); every query has >= kk >= m
+    # candidates, so its first m entries are its answer
+    o = np.lexsort((cols, cd, rows))
+    rows, cols, cd = rows[o], cols[o], cd[o]
+    start = np.searchsorted(rows, np.arange(nb))
+    take = start[:, None] + np.arange(m)[None, :]
+    return cd[take], cols[take].astype(np.int64)
 
 
 # ----------------------------------------------------------------------------
@@ -260,17 +284,18 @@ def near_tie_report(X, Y, idx, ref_dist, ref_idx, boundary, eps_rel):
     gap_lo[:, 1:] = full[:, 1:k] - full[:, :k - 1]
     gap_hi[:, :] = full[:, 1:k + 1] - full[:, :k]
     ambiguous = (gap_lo <= tol[:, :k]) | (gap_hi <= tol[:, :k])
+    safe_idx = np.clip(idx, 0, Y.shape[0] - 1)
+    diff = idx != ref_idx
+    rows = np.nonzero(diff.any(axis=1))[0]
     bad = 0
     worst = 0.0
-    safe_idx = np.clip(idx, 0, Y.shape[0] - 1)
-    got = _direct_sqdist(X, Y, safe_idx)
-    for r in range(nq):
-        for j in range(k):
-            if idx[r, j] == ref_idx[r, j]:
-                continue
-            g = abs(got[r, j] - ref_dist[r, j])
-            if ambiguous[r, j] and g <= tol[r, j] and 0 <= idx[r, j] < Y.shape[0]:
-                worst = max(worst, g)
-            else:
-                bad += 1
+    if rows.size:
+        got = _direct_sqdist(X[rows], Y, safe_idx[rows])
+        d = diff[rows]
+        g = np.abs(got - ref_dist[rows])
+        in_range = (idx[rows] >= 0) & (idx[rows] < Y.shape[0])
+        ok = ambiguous[rows] & (g <= tol[rows, :k]) & in_range
+        bad = int(np.sum(d & ~ok))
+        if np.any(d & ok):
+            worst = float(g[d & ok].max())
     return bad, int(ambiguous.sum()), worst

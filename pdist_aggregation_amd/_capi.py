@@ -23,7 +23,7 @@ PRECISION = {"exact": 0, "fp32": 0, "fp64": 0, "bf16": 1, "fp8": 2}
 FLAG_SQUARED = 1
 FLAG_TIMING = 2
 KERNEL_PATH = {0: "generic-simt", 1: "mfma-fp32", 2: "mfma-bf16", 3: "mfma-fp8", 4: "lowd-simt"}
-ABI_VERSION = 2  # knn_abi_version() of the library this binding matches
+ABI_VERSION = 3  # knn_abi_version() of the library this binding matches
 
 # Every symbol include/pdist_knn.h declares.
 EXPORTED = (
@@ -51,6 +51,7 @@ class KnnStats(ctypes.Structure):
         ("search_ms", ctypes.c_double),
         ("merge_ms", ctypes.c_double),
         ("n_rescued", ctypes.c_int64),
+        ("n_widened", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -30,6 +30,17 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+// Testing switches (PDIST_KNN_GUESS, PDIST_KNN_SPLITS, PDIST_KNN_WIDEN) are
+// read only when PDIST_KNN_TESTING=1 is set as well: they force rare code
+// paths (re-search, two-level merge, widened re-rank) at sizes a test can
+// check, and are not a supported interface.  Without the opt-in a call never
+// looks at them.
+const char* test_switch(const char* name) {
+  const char* on = std::getenv("PDIST_KNN_TESTING");
+  if (!on || std::strcmp(on, "1") != 0) return nullptr;
+  return std::getenv(name);
+}
+
 constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
@@ -84,12 +95,16 @@ struct Plan {
   int sample_regions = 0, sample_j = 0;
   int64_t sample_rows = 0, sample_stride = 0;
   double sample_fraction = 0.0, guess_fail_bound = 0.0;
+  // Widening (MFMA paths): per-query collect lists of coll_cap row indices
+  // for queries the error bound does not certify.
+  int coll_cap = 0;
   // workspace offsets
   size_t off_scalars = 0, off_ynorm = 0, off_tau = 0, off_cand_d = 0, off_cand_i = 0,
          off_part_d = 0, off_part_i = 0, off_mu = 0, off_xq = 0, off_yq = 0, off_xnorm = 0,
          off_xerr = 0, off_xscale = 0, off_yscale = 0, off_slots = 0, off_sample = 0,
          off_guess = 0, off_failed = 0, off_failed_qb = 0, off_uncert = 0, off_xpad = 0,
-         off_ypad = 0, total = 0;
+         off_ypad = 0, off_widen = 0, off_widen_qb = 0, off_ubound = 0, off_coll_cnt = 0,
+         off_coll_idx = 0, total = 0;
 };
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
@@ -138,7 +153,7 @@ int choose_splits(int64_t n_qblocks, int64_t n_train, int row_step, int kc, int6
 // no faster search and 2.7 -> 6.2 ms of merge).  PDIST_KNN_SPLITS overrides.
 int choose_splits_slots(int64_t n_qblocks, int64_t n_train, int row_step, int kc,
                         int64_t slots) {
-  if (const char* env = std::getenv("PDIST_KNN_SPLITS")) {
+  if (const char* env = test_switch("PDIST_KNN_SPLITS")) {
     const int v = std::atoi(env);
     if (v > 0) return (int)std::min<int64_t>(v, merge_split_limit(kc));
   }
@@ -200,11 +215,17 @@ static_assert(kGenericSampleCap >= knn::kSampleJ + knn::kGenY &&
 constexpr int kSampleShift = 5;  // sample 1/32 of the train rows
 constexpr double kGuessFail = 1e-7;
 constexpr int kGuessMaxJ = 128;
+// Collect lists: room for 8 k' rows per query, at least 1024 and at most
+// 16384.  A CPU model of the quantisation puts the bound-following set at
+// about k + 2 rows for BF16 on the 1e4 x 1e6 x 128 U[0,1) benchmark inputs,
+// 100-350 rows for FP8 there, and about 2,000 (a whole Gaussian) for FP8 on
+// the 1,000-Gaussian mixture in d = 256 with k' = 400 (capacity 4096).
+constexpr int kCollMin = 1024, kCollMax = 16384;
 
 // runs: sampled runs per (query, region) the kernel writes (2 for the MFMA
 // kernel's two lane halves, 1 for the low-d kernel's one lane per query).
 void plan_guess(Plan& p, int64_t slots, int runs) {
-  const char* env = std::getenv("PDIST_KNN_GUESS");
+  const char* env = test_switch("PDIST_KNN_GUESS");
   const std::string mode = env ? env : "";
   int fail_stride = 0;
   if (mode == "fail") fail_stride = 1;
@@ -408,6 +429,14 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     p.off_xpad = take((size_t)nq1 * p.dpad * sizeof(float));
     p.off_ypad = take((size_t)n_train * p.dpad * sizeof(float));
   }
+  if (mfma) {
+    p.coll_cap = std::min(kCollMax, knn::next_pow2(std::max(kCollMin, 8 * p.kc)));
+    p.off_widen = take((size_t)nq1 * sizeof(int));
+    p.off_widen_qb = take((size_t)p.n_qblocks * sizeof(int));
+    p.off_ubound = take((size_t)nq1 * sizeof(double));
+    p.off_coll_cnt = take((size_t)nq1 * sizeof(int));
+    p.off_coll_idx = take((size_t)nq1 * p.coll_cap * sizeof(int));
+  }
   if (p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8) {
     const size_t qe = p.path == kPathMfmaBF16 ? 2 : 1;
     p.off_mu = take((1 + (size_t)knn::column_mean_blocks(std::min<int64_t>(n_train, 65536))) *
@@ -513,6 +542,10 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.qblock_queries = 1;
   ma.tau = tau;
   ma.only_queries = nullptr;
+  ma.widen = nullptr;
+  ma.widen_qblocks = nullptr;
+  ma.ubound = nullptr;
+  ma.widen_all = 0;
   knn::MfmaArgs fa;
   ma.xnorm = nullptr;
   ma.xerr = nullptr;
@@ -598,6 +631,19 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     fa.qblock_active = nullptr;
     fa.query_active = nullptr;
     fa.sample_out = nullptr;
+    fa.coll_cnt = nullptr;
+    fa.coll_idx = nullptr;
+    fa.coll_cap = 0;
+    // Widening flags start clear; the merges set them.
+    ma.widen = reinterpret_cast<int*>(ws + p.off_widen);
+    ma.widen_qblocks = reinterpret_cast<int*>(ws + p.off_widen_qb);
+    ma.ubound = reinterpret_cast<double*>(ws + p.off_ubound);
+    ma.qblock_queries = knn::mfma_block_queries(
+        p.path == kPathMfmaF32 ? knn::kFmtF32 : p.path == kPathMfmaBF16 ? knn::kFmtBF16 : knn::kFmtFP8,
+        p.dm);
+    if (const char* w = test_switch("PDIST_KNN_WIDEN")) ma.widen_all = std::strcmp(w, "all") == 0;
+    knn::launch_fill_u32(reinterpret_cast<unsigned*>(ma.widen), n_test, 0u, stream);
+    knn::launch_fill_u32(reinterpret_cast<unsigned*>(ma.widen_qblocks), p.n_qblocks, 0u, stream);
     fa.cand_d = reinterpret_cast<float*>(ws + p.off_cand_d);
     fa.cand_i = reinterpret_cast<int*>(ws + p.off_cand_i);
     fa.tau = tau;
@@ -768,6 +814,38 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       rm.only_queries = ma.failed;
       merge(rm);
     }
+    // 6. queries the error bound does not certify: gather every row that can
+    // lie within their k-th exact distance (collect launch at the fixed
+    // threshold the merge set), then re-rank all of them exactly.
+    if (ma.widen) {
+      knn::MfmaArgs ca = fa;
+      ca.qblock_active = ma.widen_qblocks;
+      ca.query_active = ma.widen;
+      ca.coll_cnt = reinterpret_cast<int*>(ws + p.off_coll_cnt);
+      ca.coll_idx = reinterpret_cast<int*>(ws + p.off_coll_idx);
+      ca.coll_cap = p.coll_cap;
+      knn::launch_fill_u32(reinterpret_cast<unsigned*>(ca.coll_cnt), n_test, 0u, stream);
+      knn::launch_mfma_collect(ca, stream);
+      knn::CollectArgs<T> cr;
+      cr.X = X_test;
+      cr.Y = X_train;
+      cr.n_q = n_test;
+      cr.n_t = n_train;
+      cr.d = (int)dim;
+      cr.k = (int)k;
+      cr.squared = ma.squared;
+      cr.widen = ma.widen;
+      cr.coll_cnt = ca.coll_cnt;
+      cr.coll_idx = ca.coll_idx;
+      cr.coll_cap = p.coll_cap;
+      cr.ubound = ma.ubound;
+      cr.surv_cap = std::min(4096, knn::next_pow2(std::max(256, 2 * (int)k + 64)));
+      cr.uncert = ma.uncert;
+      cr.dist_out = dist_out;
+      cr.idx_out = idx_out;
+      knn::launch_collect_rerank<T>(cr, stream);
+      if (stats) knn::launch_count_nonzero_i32(ma.widen, n_test, &sc->n_widened, stream);
+    }
   }
   // Every query's final merge (the first one, or the re-search's for a query
   // whose guess failed) has set its flag.
@@ -789,6 +867,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     }
     stats->n_uncertified = host_sc.n_uncertified;
     stats->n_rescued = host_sc.n_rescued;
+    stats->n_widened = host_sc.n_widened;
     if (timer.on) {
       stats->prep_ms = timer.elapsed(0, 1);
       stats->search_ms = timer.elapsed(1, 2);
@@ -907,6 +986,6 @@ size_t knn_argkmin_workspace_size(int dtype_bytes, int64_t n_train, int64_t n_te
 
 const char* knn_last_error(void) { return g_last_error.c_str(); }
 
-int knn_abi_version(void) { return 2; }
+int knn_abi_version(void) { return 3; }
 
 }  // extern "C"

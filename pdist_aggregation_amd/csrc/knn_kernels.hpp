@@ -24,6 +24,8 @@ struct DeviceScalars {
   unsigned long long max_yerr_key;   // ordered key of max_j |y_j - q(y_j)| (low precision)
   unsigned int n_uncertified;
   unsigned int n_rescued;            // queries searched again after a too-tight sampled threshold
+  unsigned int n_widened;            // queries re-ranked over their bound-following row set
+  unsigned int pad_;
 };
 
 // Everything the fused search kernels need.  T is the element type of the
@@ -108,6 +110,15 @@ struct MfmaArgs {
   // Sampling launch: [n_q][n_splits][2][kSampleJ] ascending smallest
   // per-tile minima seen by each lane.
   float* sample_out;
+  // Collect launch (bound-following re-rank set): for the query blocks and
+  // queries selected by qblock_active / query_active, the index of every
+  // train row whose value is <= the query's fixed threshold tau[q] goes to
+  // coll_idx[q * coll_cap + slot], slots taken from coll_cnt[q] (zeroed
+  // before the launch); the count keeps growing past coll_cap, the entries
+  // past it are dropped.
+  int* coll_cnt;
+  int* coll_idx;
+  int coll_cap;
 };
 
 constexpr int kSlotJ = 4;
@@ -117,6 +128,8 @@ constexpr int kSampleJ = 16;  // per-lane minima kept by the sampling launch
 bool launch_mfma_search(const MfmaArgs& a, hipStream_t s);
 // The re-search launch (qblock_active and query_active must be set).
 bool launch_mfma_research(const MfmaArgs& a, hipStream_t s);
+// The collect launch (qblock_active, query_active and the coll_* fields set).
+bool launch_mfma_collect(const MfmaArgs& a, hipStream_t s);
 // The sampling launch of the same kernel: rows [s * split_stride, + rows_per_split)
 // of each split s, writing sample_out; no candidate lists.
 bool launch_mfma_sample(const MfmaArgs& a, hipStream_t s);
@@ -239,10 +252,48 @@ struct MergeArgs {
   T* pre_d;
   int* pre_i;
   int group_size, n_groups;
+  // Widening (widen == nullptr: off).  A query its final merge does not
+  // certify gets widen[q] = 1, widen_qblocks[q / qblock_queries] = 1,
+  // ubound[q] = its k-th exact squared distance U, and tau[q] = the key of T,
+  // the largest value the search can compute for a row within distance
+  // sqrt(U) of the query: a collect launch then gathers every such row.
+  // widen_all (testing) widens every query that is not trivially certified.
+  int* widen;
+  int* widen_qblocks;
+  double* ubound;
+  int widen_all;
 };
 
 template <class T>
 void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s);
+
+// Exact re-rank of the collected sets (knn_merge.hip).  For every query with
+// widen[q] != 0 whose coll_cnt[q] <= coll_cap: FP64 direct-difference
+// distances of all its collected rows, those within ubound[q] (plus a
+// relative 1e-12) kept -- at most surv_cap, a power of two -- sorted by
+// (distance, index), the first k written to the outputs and uncert[q]
+// cleared.  A query whose set overflowed keeps its first answer and flag.
+template <class T>
+struct CollectArgs {
+  const T* X;
+  const T* Y;
+  int64_t n_q, n_t;
+  int d, k;
+  int squared;
+  const int* widen;
+  const int* coll_cnt;
+  const int* coll_idx;
+  int coll_cap;
+  const double* ubound;
+  int surv_cap;
+  unsigned char* uncert;
+  T* dist_out;
+  int64_t* idx_out;
+};
+template <class T>
+void launch_collect_rerank(const CollectArgs<T>& a, hipStream_t s);
+// *out += number of nonzero entries of f[0, n).
+void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s);
 // sc->n_uncertified += number of nonzero flags in uncert[0, n).
 void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
                               hipStream_t s);

@@ -16,6 +16,13 @@ namespace knn {
 
 namespace {
 
+// The smallest float above a finite v.
+__device__ inline float next_up(float v) {
+  const unsigned b = __float_as_uint(v);
+  if (v == 0.f) return __uint_as_float(1u);  // +-0 -> smallest positive subnormal
+  return __uint_as_float(v > 0.f ? b + 1u : b - 1u);
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -504,21 +511,169 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
   // least sqrt(max(0, a_kc + |xq|^2 - delta)) - xerr - max_yerr; if that
   // exceeds the k-th exact distance the k returned rows are the exact answer.
   if (tid == 0) {
-    bool certified = a.kc >= a.n_t;
+    const bool trivial = a.kc >= a.n_t;
+    bool certified = trivial;
+    const double nx = a.xnorm ? (double)a.xnorm[q] : xnorm_s;
+    const double ny = from_ordered_key(a.sc->max_ynorm_key);
+    const double nn = (double)a.acc_terms;
+    const double gamma = nn * a.unit_roundoff / (1.0 - nn * a.unit_roundoff);
+    const double ex = a.xerr ? (double)a.xerr[q] : 0.0;
+    const double ey = a.xerr ? from_ordered_key(a.sc->max_yerr_key) : 0.0;
     if (!certified && a.kc >= a.k && full_s) {
       const double akc = akc_s;
-      const double nx = a.xnorm ? (double)a.xnorm[q] : xnorm_s;
-      const double ny = from_ordered_key(a.sc->max_ynorm_key);
-      const double nn = (double)a.acc_terms;
-      const double gamma = nn * a.unit_roundoff / (1.0 - nn * a.unit_roundoff);
       const double delta = gamma * (ny + 2.0 * sqrt(nx * ny) + fabs(akc) + nx);
-      const double ex = a.xerr ? (double)a.xerr[q] : 0.0;
-      const double ey = a.xerr ? from_ordered_key(a.sc->max_yerr_key) : 0.0;
       const double lo = akc + nx - delta;
       const double lower = sqrt(lo > 0.0 ? lo : 0.0) - ex - ey;
       certified = lower > sqrt(kth_s);
     }
+    if (a.widen_all && !trivial) certified = false;
     a.uncert[q] = certified ? 0 : 1;
+    // Widening: a train row y within the k-th exact distance sqrt(U) of x
+    // (every true neighbour is: U bounds the true k-th distance from above)
+    // has |q(x) - q(y)| <= sqrt(U) + xerr + max_yerr = R, i.e. an exact
+    // search value |q(y)|^2 - 2 q(x).q(y) <= R^2 - |q(x)|^2, and the computed
+    // one exceeds that by at most gamma (2 |q(y)|^2 + 2 |q(x)||q(y)| +
+    // |q(x)|^2) (the fp32 norms and the dot product).  T, rounded up to
+    // float (thr below), is the collect launch's fixed threshold.
+    if (!certified && a.widen && kth_s < __builtin_huge_val()) {
+      const double R = sqrt(kth_s * (1.0 + 1e-12)) + ex + ey;
+      const double thr = R * R - nx + gamma * (2.0 * ny + 2.0 * sqrt(nx * ny) + nx) +
+                         1e-300;
+      float tf = (float)thr;
+      if ((double)tf < thr) tf = next_up(tf);
+      a.widen[q] = 1;
+      atomicOr(&a.widen_qblocks[q / a.qblock_queries], 1);
+      a.ubound[q] = kth_s;
+      a.tau[q] = ordered_key((double)tf);
+    }
+  }
+}
+
+// One workgroup per query selected for widening: exact FP64 distances of
+// every collected row (tpc lanes per row, kU rows per lane group in flight),
+// those within the first answer's k-th distance U (+ 1e-12 relative: U may
+// come from another summation order) compacted into LDS, bitonic-sorted by
+// (distance, index), the first k written.  The collected set holds every
+// row within sqrt(U), hence every true neighbour and every tie at the true
+// k-th distance, so the result is exact; the query's flag is cleared.
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void collect_rerank_kernel(CollectArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char dyn_smem[];
+  const int64_t q = blockIdx.x;
+  if (a.widen[q] == 0) return;
+  const int cnt = a.coll_cnt[q];
+  if (cnt > a.coll_cap) return;  // set overflowed: the first answer and its flag stay
+  double* sd = reinterpret_cast<double*>(dyn_smem);
+  int* si = reinterpret_cast<int*>(sd + a.surv_cap);
+  __shared__ int n_s, over_s;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    n_s = 0;
+    over_s = 0;
+  }
+  __syncthreads();
+  const double cut = a.ubound[q] * (1.0 + 1e-12);
+  const int* ci = a.coll_idx + (size_t)q * a.coll_cap;
+  constexpr int kU = 4;
+  const T* xr = a.X + q * a.d;
+  using V4 = typename Vec4<T>::type;
+  const bool vec4 = (a.d & 3) == 0 &&
+                    ((reinterpret_cast<uintptr_t>(a.X) | reinterpret_cast<uintptr_t>(a.Y)) &
+                     (4 * sizeof(T) - 1)) == 0;
+  const int span = vec4 ? (a.d >> 2) : a.d;  // loads per row
+  int tpc = 16;
+  while (tpc > 1 && tpc / 2 >= span) tpc >>= 1;  // no more lanes than loads
+  const int groups = NT / tpc;
+  const int grp_j = tid / tpc, c = tid % tpc;
+  for (int j0 = 0; j0 < cnt; j0 += groups * kU) {
+    int cid[kU];
+    bool valid[kU];
+    const T* yr[kU];
+    double sum[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int j = j0 + u * groups + grp_j;
+      cid[u] = j < cnt ? ci[j] : -1;
+      valid[u] = cid[u] >= 0 && cid[u] < a.n_t;
+      yr[u] = a.Y + (int64_t)(valid[u] ? cid[u] : 0) * a.d;
+      sum[u] = 0.0;
+    }
+    if (vec4) {
+      for (int p = c; p < span; p += tpc) {
+        const V4 xv = reinterpret_cast<const V4*>(xr)[p];
+        V4 yv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) yv[u] = reinterpret_cast<const V4*>(yr[u])[p];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const double d0 = (double)xv.x - (double)yv[u].x, d1 = (double)xv.y - (double)yv[u].y;
+          const double d2 = (double)xv.z - (double)yv[u].z, d3 = (double)xv.w - (double)yv[u].w;
+          sum[u] = fma(d0, d0, sum[u]);
+          sum[u] = fma(d1, d1, sum[u]);
+          sum[u] = fma(d2, d2, sum[u]);
+          sum[u] = fma(d3, d3, sum[u]);
+        }
+      }
+    } else {
+      for (int p = c; p < span; p += tpc) {
+        const double x = (double)xr[p];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const double df = x - (double)yr[u][p];
+          sum[u] = fma(df, df, sum[u]);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      for (int o = tpc >> 1; o > 0; o >>= 1) sum[u] += __shfl_xor(sum[u], o, 64);
+    if (c == 0) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (valid[u] && sum[u] <= cut) {
+          const int slot = atomicAdd(&n_s, 1);
+          if (slot < a.surv_cap) {
+            sd[slot] = sum[u];
+            si[slot] = cid[u];
+          } else {
+            over_s = 1;
+          }
+        }
+    }
+  }
+  __syncthreads();
+  const int n = n_s;
+  if (over_s || n < a.k) return;  // cannot happen for a complete set; keep the flag
+  const int P = next_pow2(n);
+  for (int e = n + tid; e < P; e += NT) {
+    sd[e] = __builtin_huge_val();
+    si[e] = INT_MAX;
+  }
+  block_bitonic_sort(sd, si, P);
+  for (int j = tid; j < a.k; j += NT) {
+    const double x = sd[j];
+    a.idx_out[q * a.k + j] = (int64_t)si[j];
+    a.dist_out[q * a.k + j] = (T)(a.squared ? x : sqrt(x));
+  }
+  if (tid == 0) a.uncert[q] = 0;
+}
+
+__global__ __launch_bounds__(kThreads) void count_nonzero_i32_kernel(const int* __restrict__ f,
+                                                                     int64_t n,
+                                                                     unsigned int* out) {
+  __shared__ unsigned int wsum[kThreads / 64];
+  unsigned int c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+    c += f[i] != 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int t = 0;
+    for (int i = 0; i < kThreads / 64; ++i) t += wsum[i];
+    if (t) atomicAdd(out, t);
   }
 }
 
@@ -602,6 +757,21 @@ void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s) {
                        dim3(kMergeThreads), lds, s, a);
 }
 
+template <class T>
+void launch_collect_rerank(const CollectArgs<T>& a, hipStream_t s) {
+  if (a.n_q <= 0) return;
+  const size_t lds = (size_t)a.surv_cap * (sizeof(double) + sizeof(int));
+  hipLaunchKernelGGL((collect_rerank_kernel<T, kThreads>), dim3((unsigned)a.n_q), dim3(kThreads),
+                     lds, s, a);
+}
+
+void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>(64, ceil_div(n, (int64_t)kThreads));
+  hipLaunchKernelGGL(count_nonzero_i32_kernel, dim3((unsigned)std::max<int64_t>(1, blocks)),
+                     dim3(kThreads), 0, s, f, n, out);
+}
+
 void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
                               hipStream_t s) {
   if (n <= 0) return;
@@ -615,5 +785,7 @@ template void launch_row_norms<float>(const float*, int64_t, int, float*, Device
 template void launch_row_norms<double>(const double*, int64_t, int, double*, DeviceScalars*, hipStream_t);
 template void launch_merge_rerank<float>(const MergeArgs<float>&, hipStream_t);
 template void launch_merge_rerank<double>(const MergeArgs<double>&, hipStream_t);
+template void launch_collect_rerank<float>(const CollectArgs<float>&, hipStream_t);
+template void launch_collect_rerank<double>(const CollectArgs<double>&, hipStream_t);
 
 }  // namespace knn

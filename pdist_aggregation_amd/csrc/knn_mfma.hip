@@ -155,10 +155,14 @@ __device__ inline void wait_barrier_upto(int n, std::integer_sequence<int, I...>
 // separate instantiation, so profiles tell the two launches apart).  MODE
 // kModeSample: the same sweep over sampled row ranges, keeping only each
 // lane's kSampleJ smallest per-tile minima in registers (a threshold
-// estimate, no lists).
+// estimate, no lists).  MODE kModeCollect: the selected queries keep their
+// threshold fixed (tau[q], set by the merge) and every row at or below it is
+// gathered into the query's global collect list: no compaction, no threshold
+// refresh, no per-split lists.
 constexpr int kModeSearch = 0;
 constexpr int kModeSample = 1;
 constexpr int kModeResearch = 2;
+constexpr int kModeCollect = 3;
 
 template <int FMT, int DM, int QG, int MODE>
 __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) void mfma_search_kernel(MfmaArgs a) {
@@ -199,7 +203,8 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     split = w_item / a.n_qblocks;
     qb = w_item % a.n_qblocks;
   }
-  if (MODE == kModeResearch && a.qblock_active[qb] == 0) return;  // block not selected
+  constexpr bool kSelected = MODE == kModeResearch || MODE == kModeCollect;
+  if (kSelected && a.qblock_active[qb] == 0) return;  // block not selected
   const int64_t q0 = (int64_t)qb * BQ;
   const int64_t ybeg = (int64_t)split * a.split_stride;
   const int64_t yend = ybeg + a.rows_per_split < a.n_t ? ybeg + a.rows_per_split : a.n_t;
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   for (int g = 0; g < QG; ++g) {
     const int64_t q = q0 + wq0 + g * 32 + c;
     const bool qok = q < a.n_q;
-    qact[g] = qok && (MODE != kModeResearch || a.query_active[q] != 0);
+    qact[g] = qok && (!kSelected || a.query_active[q] != 0);
     tau[g] = qact[g] ? FLT_MAX : -__builtin_huge_valf();
     if constexpr (FMT == kFmtF32) {
       const float* xrow = reinterpret_cast<const float*>(a.X) + q * a.ldx;
@@ -481,6 +486,42 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
       tau[g] = fminf(tau[g], tau_s[ql]);
     }
   };
+  // Collect launch: move the entries of every query of this wave with a half
+  // above `limit` to the query's global collect list (one atomic per query
+  // and flush reserves the slots) and empty its halves.
+  auto flush = [&](int limit) {
+#pragma unroll
+    for (int g = 0; g < QG; ++g) cnt_s[2 * (wq0 + g * 32 + c) + h] = cnt[g];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    wave_sync();
+    for (int qq = 0; qq < WQ; ++qq) {
+      const int ql = wq0 + qq;
+      const int n0 = cnt_s[2 * ql], n1 = cnt_s[2 * ql + 1];
+      if (n0 <= limit && n1 <= limit) continue;  // uniform across the wave
+      // Only queries of the launch collect (a padding lane past n_q, threshold
+      // -inf, appends only -inf values, i.e. never for finite inputs).
+      const int64_t q = q0 + ql;
+      if (q < a.n_q) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&a.coll_cnt[q], n0 + n1);
+        base = __shfl(base, 0, 64);
+        const int* ci = cand_i + (size_t)ql * a.cap;
+        int* dst = a.coll_idx + (size_t)q * a.coll_cap;
+        for (int e = lane; e < n0 + n1; e += 64) {
+          const int src = e < n0 ? e : half + (e - n0);
+          if (base + e < a.coll_cap) dst[base + e] = ci[src];
+        }
+      }
+      if (lane == 0) {
+        cnt_s[2 * ql] = 0;
+        cnt_s[2 * ql + 1] = 0;
+      }
+      wave_sync();
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+#pragma unroll
+    for (int g = 0; g < QG; ++g) cnt[g] = cnt_s[2 * (wq0 + g * 32 + c) + h];
+  };
   // Appends go through buffer stores on this workgroup's candidate block, so
   // the address of an append is one 32-bit offset.
   const int cand_bytes = BQ * a.cap * 4;
@@ -530,17 +571,24 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
             const float v = acc[g][4 * r4 + e];
             const bool hit = v <= tau[g];
             const int off = hit ? base + n * 4 : kOOB;
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), cdrs, off, 0, 0);
+            if constexpr (MODE != kModeCollect)  // a collect list keeps indices only
+              __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), cdrs, off, 0, 0);
             __builtin_amdgcn_raw_buffer_store_b32(yb + r4 * 8 + e, cirs, off, 0, 0);
             n += hit;
-            n_stores += 2;
+            n_stores += MODE == kModeCollect ? 1 : 2;
           }
         }
         cnt[g] = n;
         overflow |= n > thr;
       }
     }
-    if (__any(overflow)) compact(thr);
+    if (__any(overflow)) {
+      if constexpr (MODE == kModeCollect)
+        flush(thr);
+      else
+        compact(thr);
+    }
+    if constexpr (MODE == kModeCollect) return;  // fixed threshold
     // Refresh at steps 3, 7, 15, then every 32 (cheap early sharing, rare later).
     if ((stp < 32 && ((stp + 1) & stp) == 0 && stp >= 3) || (stp & 31) == 31) {
       // Thresholds shared across the splits of a query:
@@ -672,6 +720,10 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   }
 
   sweep(select);
+  if constexpr (MODE == kModeCollect) {
+    flush(0);  // whatever is left
+    return;
+  }
 
   // ---- final: sort every list of this wave, write the split's slot --------
 #pragma unroll
@@ -682,7 +734,7 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     const int ql = wq0 + qq;
     const int64_t q = q0 + ql;
     if (q >= a.n_q) break;
-    if (MODE == kModeResearch && a.query_active[q] == 0) continue;  // uniform across the wave
+    if (kSelected && a.query_active[q] == 0) continue;  // uniform across the wave
     const int n0 = cnt_s[2 * ql], n1 = cnt_s[2 * ql + 1];
     const int n = n0 + n1;
     if (n > 0) sort_query(ql, n0, n1);
@@ -698,7 +750,8 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
 
 template <int FMT, int DM, int QG, int MODE>
 void launch_one(const MfmaArgs& a, hipStream_t s) {
-  const size_t lds = MODE == kModeSample ? 0 : (size_t)4 * a.cap * 8;
+  // the list sorts' LDS (collect lists are never sorted in the kernel)
+  const size_t lds = MODE == kModeSample || MODE == kModeCollect ? 0 : (size_t)4 * a.cap * 8;
   const unsigned blocks = (unsigned)((a.n_qblocks * a.n_splits + 7) / 8 * 8);
   hipLaunchKernelGGL((mfma_search_kernel<FMT, DM, QG, MODE>), dim3(blocks), dim3(kThreads), lds,
                      s, a);
@@ -774,6 +827,11 @@ bool launch_mfma_sample(const MfmaArgs& a, hipStream_t s) { return launch_mode<k
 
 bool launch_mfma_research(const MfmaArgs& a, hipStream_t s) {
   return a.qblock_active && a.query_active && launch_mode<kModeResearch>(a, s);
+}
+
+bool launch_mfma_collect(const MfmaArgs& a, hipStream_t s) {
+  return a.qblock_active && a.query_active && a.coll_cnt && a.coll_idx && a.coll_cap > 0 &&
+         launch_mode<kModeCollect>(a, s);
 }
 
 }  // namespace knn

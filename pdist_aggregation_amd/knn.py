@@ -71,7 +71,6 @@ def _torch():
 
 _UPLOAD_MIN_BYTES = 4 << 20  # host arrays from this size go through pinned staging
 _UPLOAD_CHUNK_BYTES = 8 << 20
-_PINNED_RESULT_MAX_BYTES = 1 << 30  # dense matrices above this come back pageable
 
 
 def _upload(src, torch, device, stream):
@@ -88,7 +87,8 @@ def _upload(src, torch, device, stream):
         with torch.cuda.stream(stream):
             return src.to(device)
     flat = src.reshape(-1)
-    out = torch.empty(src.shape, dtype=src.dtype, device=device)
+    with torch.cuda.stream(stream):
+        out = torch.empty(src.shape, dtype=src.dtype, device=device)
     dst = out.view(-1)
     n = flat.numel()
     step = _UPLOAD_CHUNK_BYTES // flat.element_size()
@@ -107,6 +107,17 @@ def _upload(src, torch, device, stream):
     return out
 
 
+def _call_stream(torch, device, stream):
+    """The stream a call runs on.  A stream other than the current one first
+    waits for the work already queued on the current stream (which may be
+    producing the inputs)."""
+    cur = torch.cuda.current_stream(device)
+    if stream is None or stream == cur:
+        return cur
+    stream.wait_stream(cur)
+    return stream
+
+
 def _as_device(a, name, torch, device, stream):
     """Return (tensor on GPU, came_from_host)."""
     if isinstance(a, torch.Tensor):
@@ -116,6 +127,9 @@ def _as_device(a, name, torch, device, stream):
             raise ValueError(f"{name} is not C-contiguous")
         if a.device.type != "cuda":
             return _upload(a, torch, device, stream), True
+        if stream != torch.cuda.current_stream(a.device):
+            # the caller may free it on its own stream while `stream` reads it
+            a.record_stream(stream)
         return a, False
     arr = np.asarray(a)
     if arr.ndim != 2:
@@ -128,18 +142,57 @@ def _as_device(a, name, torch, device, stream):
 def _to_host(tensors, torch, stream):
     """numpy copies of device tensors, through pinned host buffers.
 
-    The copies are queued on ``stream`` (after the search) into page-locked
-    buffers from torch's caching host allocator, which are reused from call to
-    call; pageable copies of the results ran at a few GB/s (1e5 x 100 float64
-    distances + int64 indices: about 40 ms).  The arrays keep their buffers
-    alive.
+    The copies are queued on ``stream`` (after the search).  Pageable copies
+    of the results ran at a few GB/s (1e5 x 100 float64 distances + int64
+    indices: about 40 ms).  Results up to _UPLOAD_CHUNK_BYTES land in
+    page-locked buffers of their own size from torch's caching host
+    allocator (kept for reuse by later calls); larger ones stream through two
+    such buffers of _UPLOAD_CHUNK_BYTES into pageable numpy arrays, so the
+    page-locked memory a process keeps stays bounded whatever the result size.
     """
-    hosts = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True) for t in tensors]
+    outs = [None] * len(tensors)
+    small = [i for i, t in enumerate(tensors)
+             if t.numel() * t.element_size() <= _UPLOAD_CHUNK_BYTES]
+    hosts = {i: torch.empty(tensors[i].shape, dtype=tensors[i].dtype, pin_memory=True)
+             for i in small}
     with torch.cuda.stream(stream):
-        for h, t in zip(hosts, tensors):
-            h.copy_(t, non_blocking=True)
+        for i, h in hosts.items():
+            h.copy_(tensors[i], non_blocking=True)
+    for i, t in enumerate(tensors):
+        if i not in hosts:
+            outs[i] = _download_chunked(t, torch, stream)
     stream.synchronize()
-    return [h.numpy() for h in hosts]
+    for i, h in hosts.items():
+        outs[i] = h.numpy()
+    return outs
+
+
+def _download_chunked(t, torch, stream):
+    """Copy a contiguous device tensor to a new numpy array through two
+    alternating page-locked buffers of _UPLOAD_CHUNK_BYTES: the device copy of
+    one chunk overlaps the host copy of the previous one."""
+    flat = t.reshape(-1)
+    out = np.empty(t.shape, dtype=torch.empty(0, dtype=t.dtype).numpy().dtype)
+    dst = out.reshape(-1)
+    step = _UPLOAD_CHUNK_BYTES // flat.element_size()
+    bufs = [torch.empty(step, dtype=flat.dtype, pin_memory=True) for _ in range(2)]
+    pending = []  # (event, buffer index, start, length), oldest first
+    with torch.cuda.stream(stream):
+        for i, s0 in enumerate(range(0, flat.numel(), step)):
+            j = i & 1
+            if len(pending) == 2:  # buffer j holds the chunk before the last: drain it
+                ev, jb, a, m = pending.pop(0)
+                ev.synchronize()
+                dst[a:a + m] = bufs[jb][:m].numpy()
+            m = min(step, flat.numel() - s0)
+            bufs[j][:m].copy_(flat[s0:s0 + m], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            pending.append((ev, j, s0, m))
+    for ev, jb, a, m in pending:
+        ev.synchronize()
+        dst[a:a + m] = bufs[jb][:m].numpy()
+    return out
 
 
 def parallel_knn(X_train, X_test, k, chunk_size=CHUNK_SIZE, strategy="auto",
@@ -173,7 +226,7 @@ def parallel_knn(X_train, X_test, k, chunk_size=CHUNK_SIZE, strategy="auto",
     lib = _capi.lib()
     device = X_train.device if isinstance(X_train, torch.Tensor) and X_train.is_cuda \
         else torch.device("cuda", torch.cuda.current_device())
-    s = stream if stream is not None else torch.cuda.current_stream(device)
+    s = _call_stream(torch, device, stream)
     Ytr, host_in = _as_device(X_train, "X_train", torch, device, s)
     Xte, host_q = _as_device(X_test, "X_test", torch, device, s)
     if Ytr.dtype != Xte.dtype or Ytr.dtype not in (torch.float32, torch.float64):
@@ -194,9 +247,14 @@ def parallel_knn(X_train, X_test, k, chunk_size=CHUNK_SIZE, strategy="auto",
                                               prec, ov)
     if ws_bytes == 0:
         _capi.check(_capi.KNN_EINVAL)
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
-    dist = torch.empty((n_test, k), dtype=Ytr.dtype, device=device)
-    idx = torch.empty((n_test, k), dtype=torch.int64, device=device)
+    # Allocated on the call's stream: torch's caching allocator hands a freed
+    # block only to later work on the stream it was allocated on, so the
+    # workspace freed at return cannot be reused while the kernels on `s`
+    # still run.
+    with torch.cuda.stream(s):
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+        dist = torch.empty((n_test, k), dtype=Ytr.dtype, device=device)
+        idx = torch.empty((n_test, k), dtype=torch.int64, device=device)
     stats = _capi.KnnStats()
     fn = lib.knn_argkmin_f64 if is64 else lib.knn_argkmin_f32
     threads = effective_n_threads() if n_threads is None else int(n_threads)
@@ -237,22 +295,20 @@ def sqdist_matrix(X_train, X_test, stream=None):
     lib = _capi.lib()
     device = X_train.device if isinstance(X_train, torch.Tensor) and X_train.is_cuda \
         else torch.device("cuda", torch.cuda.current_device())
-    s = stream if stream is not None else torch.cuda.current_stream(device)
+    s = _call_stream(torch, device, stream)
     Ytr, _ = _as_device(X_train, "X_train", torch, device, s)
     Xte, _ = _as_device(X_test, "X_test", torch, device, s)
     if Ytr.dtype != Xte.dtype or Ytr.dtype not in (torch.float32, torch.float64):
         raise TypeError("X_train and X_test must both be float32 or both float64")
     if Ytr.shape[1] != Xte.shape[1]:
         raise ValueError("Dimensionality of test and train datasets must be the same")
-    out = torch.empty((Xte.shape[0], Ytr.shape[0]), dtype=Ytr.dtype, device=device)
+    with torch.cuda.stream(s):
+        out = torch.empty((Xte.shape[0], Ytr.shape[0]), dtype=Ytr.dtype, device=device)
     fn = lib.knn_sqdist_matrix_f64 if Ytr.dtype == torch.float64 else lib.knn_sqdist_matrix_f32
     with torch.cuda.device(device):
         code = fn(Ytr.data_ptr(), Ytr.shape[0], Xte.data_ptr(), Xte.shape[0], Ytr.shape[1],
                   out.data_ptr(), ctypes.c_void_p(s.cuda_stream))
     _capi.check(code)
     if not (isinstance(X_test, torch.Tensor) and X_test.is_cuda):
-        if out.numel() * out.element_size() <= _PINNED_RESULT_MAX_BYTES:
-            return _to_host([out], torch, s)[0]
-        with torch.cuda.stream(s):  # larger: no pinned block of this size kept around
-            return out.cpu().numpy()
+        return _to_host([out], torch, s)[0]
     return out

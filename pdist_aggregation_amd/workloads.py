@@ -6,8 +6,9 @@ GPU tests check (same seeds, same draw order), through the C ABI with the
 library's per-stage HIP-event timing (``KNN_FLAG_TIMING``), inputs resident
 in device memory:
 
-* ``plain-f64``: X 1,000 x 50 and Y 10,000 x 50 float64 U[0,1) from seed 0
-  (Y drawn first), k = 10 -- the reference's CPU-runnable case;
+* ``plain-f64``: X 1,000 x 50 and Y 10,000 x 50 float64 U[0,1) from
+  numpy RandomState(0) (Y drawn first), k = 10 -- the reference's
+  CPU-runnable case, the inputs of tests/golden/make_golden.py;
 * ``lowd``: 65,536 x 65,536 points on the unit sphere with N(0, 0.01^2)
   radial jitter, d = 3, k = 16, float32; forward, then forward + backward
   through ``knn_distances``;
@@ -53,9 +54,9 @@ def _sphere(rs, n):
 def _inputs(name):
     """(Y, X, k, precision, oversample) of a workload, host arrays."""
     if name == "plain-f64":
-        g = np.random.default_rng(0)
-        Y = g.random((10_000, 50))
-        X = g.random((1_000, 50))
+        rs = np.random.RandomState(0)
+        Y = rs.rand(10_000, 50)
+        X = rs.rand(1_000, 50)
         return Y, X, 10, "exact", 0
     if name == "lowd":
         rs = np.random.RandomState(0)

@@ -1,6 +1,11 @@
 import json
+import os
 import pathlib
 import sys
+
+# The library reads its testing switches (PDIST_KNN_GUESS, PDIST_KNN_SPLITS,
+# PDIST_KNN_WIDEN) only with this opt-in; the tests set them per case.
+os.environ["PDIST_KNN_TESTING"] = "1"
 
 import numpy as np
 import pytest

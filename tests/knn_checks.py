@@ -58,3 +58,41 @@ def assert_knn_parity(X_test, X_train, dist, idx, k, eps_rel, squared=True, dist
     bad, ambiguous, _ = oracle.near_tie_report(X_test, X_train, idx, rd, ri, rb, eps_rel)
     assert bad == 0, f"{bad} index mismatches outside near-ties"
     return ambiguous
+
+
+def assert_complete(X_test, X_train, idx, k, eps_rel, block=1 << 24):
+    """Every row of every query, at any size: the returned rows are sorted by
+    their exact FP64 distance, and no train row outside them lies below the
+    k-th returned distance by more than the near-tie width.
+
+    For each block of queries the FP64 expansion |x|^2 + |y|^2 - 2 x.y of
+    every (query, train row) pair is compared with c = kth - eps_rel *
+    max(1, kth) - e, where e bounds the expansion's rounding error: the count
+    of all rows below c must equal the count of returned rows below c, i.e.
+    every row whose exact distance is below kth - eps_rel * max(1, kth) is in
+    the answer.  Returns the number of queries checked.
+    """
+    X = np.asarray(X_test, np.float64)
+    Y = np.asarray(X_train, np.float64)
+    idx = np.asarray(idx)
+    nq, nt = X.shape[0], Y.shape[0]
+    assert idx.shape == (nq, k)
+    assert idx.min() >= 0 and idx.max() < nt
+    ynorm = np.einsum("ij,ij->i", Y, Y)
+    qb = max(1, block // nt)
+    for q0 in range(0, nq, qb):
+        Xb = X[q0:q0 + qb]
+        ib = idx[q0:q0 + qb]
+        exact = oracle._direct_sqdist(Xb, Y, ib)
+        assert np.all(np.diff(exact, axis=1) >= 0), "rows not sorted by exact distance"
+        xnorm = np.einsum("ij,ij->i", Xb, Xb)
+        D = xnorm[:, None] + ynorm[None, :] - 2.0 * (Xb @ Y.T)
+        kth = exact[:, -1]
+        e = 1e-12 * (xnorm + ynorm.max() + 1.0)
+        c = kth - eps_rel * np.maximum(1.0, kth) - e
+        n_all = np.sum(D < c[:, None], axis=1)
+        n_ret = np.sum(np.take_along_axis(D, ib, axis=1) < c[:, None], axis=1)
+        bad = np.nonzero(n_all != n_ret)[0]
+        assert bad.size == 0, (f"{bad.size} queries miss a train row clearly below their k-th "
+                               f"distance (first: query {q0 + bad[0]})")
+    return nq

@@ -50,10 +50,10 @@ def test_forced_splits_match_oracle(require_gpu, monkeypatch, precision, nt, nq,
     ref = oracle.exact_knn_f64(X, Y, k, return_boundary=True)
     rec = oracle.recall_at_k(idx, ref[1])
     _say(f"    recall@{k}={rec:.5f}")
-    assert rec >= (0.999 if precision == "bf16" else 0.99)
     np.testing.assert_allclose(dist, oracle._direct_sqdist(X, Y, idx), rtol=1e-6)
-    if st["n_uncertified"] == 0:
-        assert_knn_parity(X, Y, dist, idx, k, EPS_F32, ref=ref)
+    assert st["n_uncertified"] == 0
+    assert_knn_parity(X, Y, dist, idx, k, EPS_F32, ref=ref)
+    assert rec >= 0.999
 
 
 def test_forced_splits_with_sampled_guess_and_failures(require_gpu, monkeypatch):

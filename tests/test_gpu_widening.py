@@ -1,0 +1,108 @@
+"""Bound-following re-rank sets of the MFMA paths (needs a GPU).
+
+A query whose k' candidates do not prove the answer exact (the error-bound
+certificate of the merge fails) is widened: the merge sets its threshold T to
+the largest value the search can compute for any train row within its k-th
+exact distance, a collect launch gathers every row at or below T, and all of
+them are re-ranked in FP64.  PDIST_KNN_WIDEN=all (a testing switch) widens
+every query, so the path is compared with the FP64 oracle here on small
+inputs for every MFMA format, with exact ties at the k-th distance, together
+with failing sampled guesses (the re-search's merge widens too), and with a
+collect set larger than its capacity (the query keeps its first answer and
+stays counted as uncertified).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import pdist_aggregation_amd as pa
+from knn_checks import EPS_F32, assert_knn_parity, assert_sorted_unique
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _run(Y, X, k, precision, **kw):
+    (dist, idx), _, st = pa.parallel_knn(Y, X, k, return_distance=True, squared=True,
+                                         precision=precision, return_stats=True, **kw)
+    return dist, idx, st
+
+
+CASES = [("exact", 64, 10), ("exact", 130, 7), ("bf16", 128, 10), ("bf16", 50, 32),
+         ("fp8", 128, 10), ("fp8", 256, 40)]
+
+
+@pytest.mark.parametrize("precision,d,k", CASES)
+def test_widen_every_query_matches_oracle(require_gpu, monkeypatch, precision, d, k):
+    monkeypatch.setenv("PDIST_KNN_WIDEN", "all")
+    rs = np.random.RandomState(d + k)
+    Y = rs.rand(30000, d).astype(np.float32)
+    X = rs.rand(301, d).astype(np.float32)
+    _say(f"[widen all] {precision} d={d} k={k} 30000 x 301")
+    dist, idx, st = _run(Y, X, k, precision)
+    _say(f"    stats {st}")
+    assert st["n_widened"] == X.shape[0]
+    assert st["n_uncertified"] == 0
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, k, EPS_F32)
+
+
+@pytest.mark.parametrize("precision", ["exact", "bf16", "fp8"])
+def test_widen_with_exact_ties_at_the_cut(require_gpu, monkeypatch, precision):
+    """Train rows duplicated so that exact ties straddle the k-th rank: the
+    widened answer lists the lower indices first, like the oracle."""
+    monkeypatch.setenv("PDIST_KNN_WIDEN", "all")
+    rs = np.random.RandomState(21)
+    Y = rs.rand(20000, 64).astype(np.float32)
+    X = rs.rand(97, 64).astype(np.float32)
+    # for every query, 6 copies of one row placed near it: ranks 1-6 tie, k = 4
+    for q in range(X.shape[0]):
+        base = X[q] + np.float32(0.02) * rs.randn(64).astype(np.float32)
+        rows = rs.choice(20000, 6, replace=False)
+        Y[rows] = base
+    dist, idx, st = _run(Y, X, 4, precision)
+    assert st["n_uncertified"] == 0
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, 4, 1e-12)
+
+
+def test_widen_after_failed_guesses(require_gpu, monkeypatch):
+    """Every second guess fails: those queries are widened by the re-search's
+    merge, the others by the first merge."""
+    monkeypatch.setenv("PDIST_KNN_WIDEN", "all")
+    monkeypatch.setenv("PDIST_KNN_GUESS", "fail:2")
+    rs = np.random.RandomState(8)
+    Y = rs.rand(50000, 128).astype(np.float32)
+    X = rs.rand(515, 128).astype(np.float32)
+    dist, idx, st = _run(Y, X, 10, "bf16")
+    _say(f"    stats {st}")
+    assert st["n_rescued"] == (X.shape[0] + 1) // 2
+    assert st["n_widened"] == X.shape[0]
+    assert st["n_uncertified"] == 0
+    assert_knn_parity(X, Y, dist, idx, 10, EPS_F32)
+
+
+def test_collect_overflow_keeps_first_answer(require_gpu, monkeypatch):
+    """Query 0 sits on a train row repeated 3,000 times (exact ties at
+    distance 0): its row set exceeds the collect capacity (1,024 at k'=18),
+    so it keeps the first answer (the lowest indices of the repeated row, the
+    correct one) and is the one query counted as uncertified."""
+    monkeypatch.setenv("PDIST_KNN_WIDEN", "all")
+    rs = np.random.RandomState(9)
+    Y = rs.rand(40000, 32).astype(np.float32)
+    rep = np.sort(rs.choice(40000, 3000, replace=False))
+    Y[rep] = Y[rep[0]]
+    X = rs.rand(200, 32).astype(np.float32)
+    X[0] = Y[rep[0]]
+    dist, idx, st = _run(Y, X, 10, "exact")
+    _say(f"    stats {st}")
+    assert st["n_widened"] == X.shape[0]
+    assert st["n_uncertified"] == 1
+    np.testing.assert_array_equal(idx[0], rep[:10])
+    assert np.all(dist[0] == 0)
+    assert_knn_parity(X, Y, dist, idx, 10, EPS_F32)

@@ -4,17 +4,18 @@
   queries x 1e6 train rows, k=32, k'=128.
 * FP8 E4M3: 1,000 Gaussians (centres U[-1,1]^256, sigma 0.05), 5e4 queries x
   2e6 train rows with 20,000 train rows overwritten by copies of other rows
-  (exact ties), k=100, k'=400, plus the float32 backward dL/dX.  Here the
-  3-bit mantissa limits what k'=400 candidates can hold: a numpy model of
-  the same quantisation (mean-centred rows, power-of-two row scale, E4M3
-  round-to-nearest) gives recall@100 = 0.9856 on the same 50 queries (and
-  0.9994 with k'=800), so the bar is 0.98.
+  (exact ties), k=100, k'=400, plus the float32 backward dL/dX.  The 3-bit
+  mantissa makes the error bound wider than a Gaussian's spread, so no query
+  is certified from its k'=400 candidates (their recall@100 is about 0.986):
+  every query is widened to all rows within the bound of its k-th exact
+  distance (about 2,000 rows, the query's whole Gaussian) and re-ranked
+  exactly.
 
 Both draw from numpy default_rng(seed) in the order centres, train labels,
 train noise, query labels, query noise.  Checked on every row: ascending,
 unique, in range; on a query subsample: recall@k against the FP64 oracle,
-distances equal to the exact FP64 recomputation, and index parity on
-certified queries (near-ties exempt).
+distances equal to the exact FP64 recomputation, index parity on every
+query of the subsample (near-ties exempt), and no query uncertified.
 """
 import sys
 
@@ -51,15 +52,17 @@ def _check_rows(dist, idx, n_train):
     assert np.all(s[:, 1:] != s[:, :-1]), "duplicate index in a row"
 
 
-def _check_subsample(X, Y, dist, idx, k, sub, st, min_recall):
+def _check_subsample(X, Y, dist, idx, k, sub, st):
     ref = oracle.exact_knn_f64(X[sub], Y, k, return_boundary=True)
     rec = oracle.recall_at_k(idx[sub], ref[1])
     got = oracle._direct_sqdist(X[sub], Y, idx[sub])
     np.testing.assert_allclose(dist[sub], got, rtol=1e-6)
-    _say(f"    recall@{k} on {len(sub)} queries = {rec:.5f}; stats {st}")
-    assert rec >= min_recall
-    if st["n_uncertified"] == 0:
-        assert_knn_parity(X[sub], Y, dist[sub], idx[sub], k, EPS_F32, ref=ref)
+    err = np.max(np.abs(dist[sub].astype(np.float64) - ref[0]) / np.maximum(ref[0], 1e-30))
+    _say(f"    recall@{k} on {len(sub)} queries = {rec:.5f}, max rel dist err {err:.2e}; "
+         f"stats {st}")
+    assert st["n_uncertified"] == 0
+    assert_knn_parity(X[sub], Y, dist[sub], idx[sub], k, EPS_F32, ref=ref)
+    assert rec >= 0.999
 
 
 def test_bf16_mixture_1e5_by_1e6(require_gpu):
@@ -71,7 +74,7 @@ def test_bf16_mixture_1e5_by_1e6(require_gpu):
                                          precision="bf16", oversample=128, return_stats=True)
     dist, idx = dist.cpu().numpy(), idx.cpu().numpy()
     _check_rows(dist, idx, Y.shape[0])
-    _check_subsample(X, Y, dist, idx, 32, np.arange(0, 100_000, 400), st, 0.999)
+    _check_subsample(X, Y, dist, idx, 32, np.arange(0, 100_000, 200), st)
 
 
 def test_fp8_mixture_with_duplicates_5e4_by_2e6(require_gpu):
@@ -85,7 +88,7 @@ def test_fp8_mixture_with_duplicates_5e4_by_2e6(require_gpu):
                                          precision="fp8", oversample=400, return_stats=True)
     dist, idx = dist.cpu().numpy(), idx.cpu().numpy()
     _check_rows(dist, idx, Y.shape[0])
-    _check_subsample(X, Y, dist, idx, 100, np.arange(0, 50_000, 1000), st, 0.98)
+    _check_subsample(X, Y, dist, idx, 100, np.arange(0, 50_000, 250), st)
     # float32 backward: dL/dX of L = sum of squared k-NN distances is
     # 2 * sum_j (x - y_j) over the returned neighbours (the same search).
     Xg = Xd.clone().requires_grad_(True)
