@@ -97,6 +97,26 @@ int knn_argkmin_f64(const double* X_train, int64_t n_train, const double* X_test
                     size_t workspace_bytes, void* stream, knn_stats* stats);
 
 /*
+ * knn_argkmin_f32 without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * It calls no hipMalloc, hipFree or other allocating routine: its scratch is
+ * a static device array of KNN_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends.  A request needing more (knn_argkmin_workspace_size() above
+ * that size) fails with KNN_EWORKSPACE; KNN_FLAG_TIMING is rejected
+ * (KNN_EINVAL: it would create HIP events).  Every call on one GPU shares
+ * that GPU's array, so calls that overlap on one GPU (on different streams,
+ * or from different threads without ordering) race and must not be made.
+ * Arguments and results are those of knn_argkmin_f32.
+ */
+#define KNN_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int knn_argkmin_f32_noalloc(const float* X_train, int64_t n_train, const float* X_test,
+                            int64_t n_test, int64_t dim, int64_t k, int64_t chunk_size,
+                            int strategy, int precision, int64_t oversample, int flags,
+                            int n_threads_hint, float* dist_out, int64_t* idx_out, void* stream,
+                            knn_stats* stats);
+
+/*
  * Gradients of the k-NN distances returned by knn_argkmin_* with respect to
  * the train and query rows.  No counterpart in the reference (its search is
  * not differentiable); used by the torch.autograd.Function of

@@ -22,6 +22,7 @@ STRATEGY_NAME = {v: k for k, v in STRATEGY.items()}
 PRECISION = {"exact": 0, "fp32": 0, "fp64": 0, "bf16": 1, "fp8": 2}
 FLAG_SQUARED = 1
 FLAG_TIMING = 2
+NOALLOC_SCRATCH_BYTES = 128 << 20  # KNN_NOALLOC_SCRATCH_BYTES of include/pdist_knn.h
 KERNEL_PATH = {0: "generic-simt", 1: "mfma-fp32", 2: "mfma-bf16", 3: "mfma-fp8", 4: "lowd-simt"}
 ABI_VERSION = 3  # knn_abi_version() of the library this binding matches
 
@@ -29,6 +30,7 @@ ABI_VERSION = 3  # knn_abi_version() of the library this binding matches
 EXPORTED = (
     "knn_argkmin_f32",
     "knn_argkmin_f64",
+    "knn_argkmin_f32_noalloc",
     "knn_argkmin_backward_f32",
     "knn_argkmin_backward_f64",
     "knn_sqdist_matrix_f32",
@@ -88,6 +90,10 @@ def lib():
             fn = getattr(h, name)
             fn.restype = ctypes.c_int
             fn.argtypes = _argkmin_argtypes()
+        v, i64, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+        h.knn_argkmin_f32_noalloc.restype = ctypes.c_int
+        h.knn_argkmin_f32_noalloc.argtypes = [v, i64, v, i64, i64, i64, i64, i, i, i64, i, i,
+                                              v, v, v, ctypes.POINTER(KnnStats)]
         for name in ("knn_argkmin_backward_f32", "knn_argkmin_backward_f64"):
             fn = getattr(h, name)
             fn.restype = ctypes.c_int

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 
@@ -926,9 +927,76 @@ int run_sqdist(const T* X_train, int64_t n_train, const T* X_test, int64_t n_tes
   return KNN_OK;
 }
 
+// Scratch of knn_argkmin_f32_noalloc: a file-scope device array, so that
+// entry point allocates nothing.  Every GPU the library runs that entry point
+// on holds its own copy until the process ends.  Its device address is
+// looked up once per device (hipGetSymbolAddress, with that device current)
+// and only that address is ever handed to kernels or copies: in host code
+// the array's name is a host-side stand-in, not a device address.
+constexpr size_t kNoAllocScratchBytes = KNN_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address (nullptr and a message on failure).
+void* device_scratch(std::string& err) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+    return nullptr;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    err = "device ordinal " + std::to_string(dev) + " beyond the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 extern "C" {
+
+int knn_argkmin_f32_noalloc(const float* X_train, int64_t n_train, const float* X_test,
+                            int64_t n_test, int64_t dim, int64_t k, int64_t chunk_size,
+                            int strategy, int precision, int64_t oversample, int flags,
+                            int n_threads_hint, float* dist_out, int64_t* idx_out, void* stream,
+                            knn_stats* stats) {
+  g_last_error.clear();
+  if (flags & KNN_FLAG_TIMING)
+    return fail(KNN_EINVAL, "KNN_FLAG_TIMING creates HIP events: not available in the "
+                            "allocation-free entry point");
+  const size_t need =
+      plan_workspace(sizeof(float), n_train, n_test, dim, k, precision, oversample);
+  if (need == 0) {  // an invalid request: let the shared path name the reason
+    Plan p;
+    const int rc = make_plan(p, sizeof(float), n_train, n_test, dim, k, precision, oversample);
+    return rc != KNN_OK ? rc : fail(KNN_EINVAL, "invalid request");
+  }
+  if (need > kNoAllocScratchBytes)
+    return fail(KNN_EWORKSPACE, "this call needs " + std::to_string(need) +
+                                    " bytes of scratch; knn_argkmin_f32_noalloc has " +
+                                    std::to_string(kNoAllocScratchBytes) +
+                                    " (use knn_argkmin_f32 with a workspace)");
+  std::string err;
+  void* ws = device_scratch(err);
+  if (!ws) return fail(KNN_EHIP, err);
+  return run<float>(X_train, n_train, X_test, n_test, dim, k, chunk_size, strategy, precision,
+                    oversample, flags, n_threads_hint, dist_out, idx_out, ws,
+                    kNoAllocScratchBytes, stream, stats);
+}
 
 int knn_sqdist_matrix_f32(const float* X_train, int64_t n_train, const float* X_test,
                           int64_t n_test, int64_t dim, float* out, void* stream) {
