@@ -7,13 +7,15 @@
 // the query x train distance tile in HBM.
 //
 // Workgroup (qb, s) owns 64 queries and the train rows of split s.  Per step
-// it stages a 64-row train slice and the query slice through LDS (16 features
-// at a time) and every thread accumulates a 4 x 4 block of dot products: 4
-// consecutive queries x 4 consecutive rows, so each operand quadruple is one
-// or two 128-bit LDS reads (float64 1e5 x 1e3 x 500 search 5.3 -> 4.8 ms
-// against the strided 16-apart layout).  The
-// reduced distance v = |y|^2 - 2 x.y (the reference's "reduced distance",
-// :225-232) is compared with the query's threshold tau; survivors are
+// of 64 train rows it streams 16-feature slices of the queries (scaled by -2,
+// exact) and of the rows through two LDS buffers -- the next slice is loaded
+// into registers while the matrix cores work on the current one -- and runs
+// the 64 x 64 tile on v_mfma_f64_16x16x4_f64 (double) or
+// v_mfma_f32_16x16x4_f32 (float): wave w takes queries [16w, 16w + 16) and
+// the four 16-row blocks, so lane l ends with the values of four queries
+// (16w + Mfma16::row(l/16, i), i < 4) and rows 16j + l%16 (j < 4), the
+// accumulators starting at |y|^2.  The reduced distance v = |y|^2 - 2 x.y (the reference's "reduced
+// distance", :225-232) is compared with the query's threshold tau; survivors are
 // appended to the query's candidate list in global scratch.  When a list
 // could overflow on the next step, the workgroup sorts it by (v, index),
 // keeps the best k' and lowers tau to the k'-th value.  tau is also shared
@@ -36,6 +38,43 @@ namespace knn {
 namespace {
 
 constexpr int kGenSearch = 0, kGenSample = 1, kGenResearch = 2;
+
+// 16 x 16 x 4 MFMAs.  Lane l supplies A[l%16][l/16] and B[l/16][l%16] and
+// holds D rows row(l/16, r), r < 4, of column l%16 -- measured on gfx950:
+// blocks of four rows for f32 (4(l/16) + r), interleaved for f64 (l/16 + 4r).
+// Raw buffer load of one element at byte offset voff + soff (0 outside the
+// resource's range).
+template <class T>
+__device__ inline T buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ inline float buffer_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ inline double buffer_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  const u2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return __builtin_bit_cast(double, v);
+}
+
+template <class T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int g, int r) { return g + 4 * r; }
+};
+template <>
+struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int g, int r) { return 4 * g + r; }
+};
 
 // Sort and cut to k' every list of the workgroup longer than min_count: each
 // wave takes queries w, w+4, ... and sorts them alone (wave_sort_cut in its
@@ -165,11 +204,14 @@ __device__ void compact_lists_sample(const SearchArgs<T>& a, T* cd, int* ci, T* 
 }
 
 template <class T, int MODE>
-__global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> a) {
+__global__ __launch_bounds__(kThreads, 4) void generic_search_kernel(SearchArgs<T> a) {
   constexpr bool kShare = MODE != kGenSample;  // a sample's k-th value bounds nothing
   constexpr int kRowPad = 16 / sizeof(T);  // keeps every slice row 16-byte aligned
-  __shared__ __align__(16) T xs[kGenK][kGenQ + kRowPad];
-  __shared__ __align__(16) T ys[kGenK][kGenY + kRowPad];
+  using M = Mfma16<T>;
+  using V4 = typename M::v4;
+  // two slice buffers: slice t in buffer t & 1
+  __shared__ __align__(16) T xs[2][kGenK][kGenQ + kRowPad];
+  __shared__ __align__(16) T ys[2][kGenK][kGenY + kRowPad];
   __shared__ T tau_s[kGenQ];
   __shared__ int cnt_s[kGenQ];
   __shared__ int flag_s;
@@ -179,7 +221,7 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
 
   if (MODE == kGenResearch && a.qblock_active[blockIdx.x] == 0) return;  // whole workgroup
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
+  const int lane = tid & 63, w = tid >> 6;
   const int64_t q0 = (int64_t)blockIdx.x * kGenQ;
   const int64_t ybeg =
       (int64_t)blockIdx.y * (a.split_stride > 0 ? a.split_stride : a.rows_per_split);
@@ -205,50 +247,93 @@ __global__ __launch_bounds__(kThreads) void generic_search_kernel(SearchArgs<T> 
   if (tid == 0) flag_s = 0;
   __syncthreads();
 
+  // Slice loads: thread tid brings elements tid + 256 u (u < kPer) of the
+  // 64 x kGenK query and row slices (kGenK consecutive features of one row per
+  // kGenK lanes).
+  // Features past d and rows past the query block / split read as zero.
+  constexpr int kPer = kGenQ * kGenK / kThreads;
+  static_assert(kPer * kThreads == kGenQ * kGenK && kGenQ == kGenY && kGenK % 4 == 0,
+                "slice geometry");
+  const int n_slices = (a.d + kGenK - 1) / kGenK;
+  // Element u of a thread: row tid / kGenK + u * (kThreads / kGenK), feature
+  // tid % kGenK of the slice.  Loads are buffer loads: a scalar base per query
+  // block / row step, one lane offset for all u, the slice and u in scalar
+  // offsets; rows past the block or the split fall outside the buffer range
+  // and read as zero, features past d are zeroed here.
+  constexpr int kRowsPerU = kThreads / kGenK;
+  const int lr = tid / kGenK, lc = tid % kGenK;
+  const int lane_off = (lr * a.d + lc) * (int)sizeof(T);
+  const int u_off = kRowsPerU * a.d * (int)sizeof(T);
+  const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<T*>(a.X + q0 * a.d), (short)0, (int)(n_q_here * a.d * (int64_t)sizeof(T)),
+      0x00020000);
+  T xr[kPer], yr[kPer];
+  auto load_slice = [&](int64_t y0, int p0) {
+    const int64_t rows = yend - y0 < kGenY ? yend - y0 : kGenY;
+    const __amdgpu_buffer_rsrc_t yrs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T*>(a.Y + y0 * a.d), (short)0, (int)(rows * a.d * (int64_t)sizeof(T)),
+        0x00020000);
+    const bool in_d = lc + p0 < a.d;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int so = u * u_off + p0 * (int)sizeof(T);
+      const T xv = buffer_load<T>(xrs, lane_off, so);
+      const T yv = buffer_load<T>(yrs, lane_off, so);
+      xr[u] = in_d ? T(-2) * xv : T(0);
+      yr[u] = in_d ? yv : T(0);
+    }
+  };
+  auto store_slice = [&](int b) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      xs[b][lc][lr + u * kRowsPerU] = xr[u];
+      ys[b][lc][lr + u * kRowsPerU] = yr[u];
+    }
+  };
+  // MFMA operands: A = queries (lane l: query l%16 of the wave's block,
+  // feature l/16 of the 4-feature step), B = rows (row l%16 of block j).
+  const int fk = lane >> 4, fm = lane & 15;
+  if (yend > ybeg) load_slice(ybeg, 0);
+  store_slice(0);
+  __syncthreads();
+  int t = 0;  // slices computed so far (buffer t & 1 holds slice t)
+
   int step = 0;
   for (int64_t y0 = ybeg; y0 < yend; y0 += kGenY, ++step) {
-    T acc[4][4];
+    V4 acc[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const int64_t y = y0 + 16 * j + fm;
+      const T n0 = y < yend ? a.ynorm[y] : pos_inf<T>();
+      acc[j] = V4{n0, n0, n0, n0};
+    }
+    for (int sl = 0; sl < n_slices; ++sl, ++t) {
+      const int b = t & 1;
+      // prefetch the next slice (of this step, or the first of the next one)
+      const bool more = sl + 1 < n_slices || y0 + kGenY < yend;
+      if (more) load_slice(sl + 1 < n_slices ? y0 : y0 + kGenY,
+                           sl + 1 < n_slices ? (sl + 1) * kGenK : 0);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
-
-    for (int p0 = 0; p0 < a.d; p0 += kGenK) {
-      for (int e = tid; e < kGenQ * kGenK; e += kThreads) {
-        const int r = e / kGenK, c = e % kGenK;
-        const int64_t q = q0 + r;
-        const int p = p0 + c;
-        xs[c][r] = (r < n_q_here && p < a.d) ? a.X[q * a.d + p] : T(0);
-        const int64_t y = y0 + r;
-        ys[c][r] = (y < yend && p < a.d) ? a.Y[y * a.d + p] : T(0);
+      for (int k0 = 0; k0 < kGenK; k0 += 4) {
+        const T av = xs[b][k0 + fk][16 * w + fm];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = M::op(av, ys[b][k0 + fk][16 * j + fm], acc[j]);
       }
-      __syncthreads();
-#pragma unroll
-      for (int c = 0; c < kGenK; ++c) {
-        T xv[4], yv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) xv[i] = xs[c][4 * tx + i];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) yv[j] = ys[c][4 * ty + j];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = fma(xv[i], yv[j], acc[i][j]);
-      }
+      if (more) store_slice(b ^ 1);  // buffer b^1 was last read before the previous barrier
       __syncthreads();
     }
 
     // Filter: append every reduced distance <= tau to the query's list.
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int qq = 4 * tx + i;
+      const int qq = 16 * w + M::row(fk, i);
       if (qq >= n_q_here) continue;
       const T tq = tau_s[qq];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int64_t y = y0 + 4 * ty + j;
+        const int64_t y = y0 + 16 * j + fm;
         if (y >= yend) continue;
-        const T v = a.ynorm[y] - T(2) * acc[i][j];
+        const T v = acc[j][i];
         if (v <= tq) {
           const int pos = atomicAdd(&cnt_s[qq], 1);
           cd[(size_t)qq * a.cap + pos] = v;
