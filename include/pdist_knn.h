@@ -58,7 +58,9 @@ typedef struct knn_stats {
   int64_t k_candidates;      /* k' kept per query before the exact re-rank */
   int64_t n_splits;          /* train splits searched in parallel */
   int64_t strategy_used;     /* KNN_STRATEGY_CHUNK_ON_TRAIN or _ON_TEST */
-  int64_t kernel_path;       /* 0 generic SIMT, 1 MFMA fp32, 2 MFMA bf16, 3 MFMA fp8, 4 low-d SIMT */
+  int64_t kernel_path;       /* 0 16x16x4 MFMA (float64, float32 d > 256), 1 MFMA fp32,
+                                2 MFMA bf16, 3 MFMA fp8, 4 low-d SIMT, 5 large k (k above
+                                1,984: dense FP64 rows, radix select, rank sort) */
   double prep_ms;            /* KNN_FLAG_TIMING: fills, norms, quantisation, sampling */
   double search_ms;          /* KNN_FLAG_TIMING: fused distance + candidate kernel */
   double merge_ms;           /* KNN_FLAG_TIMING: merge + exact re-rank (+ re-search) */
@@ -78,7 +80,9 @@ typedef struct knn_stats {
  * C shim p_knn_search<floatT, intT> (sycl_ext/parallel_knn_helper.hpp:7-23).
  *
  * oversample: k' candidates kept before the exact re-rank (0 = default:
- * k + max(8, k/8) for EXACT, 4k for BF16/FP8).  n_threads_hint: the host
+ * k + max(8, k/8) for EXACT, 4k for BF16/FP8).  Any k <= n_train is accepted;
+ * BF16/FP8 requests beyond those kernels (dim > 256 or k' > 478) run the
+ * exact search.  n_threads_hint: the host
  * thread count the reference's "auto" rule would use (<= 0: this machine's).
  * workspace: device scratch of knn_argkmin_workspace_size() bytes, or NULL to
  * let the call allocate (stream-ordered) and free it.

@@ -23,7 +23,8 @@ PRECISION = {"exact": 0, "fp32": 0, "fp64": 0, "bf16": 1, "fp8": 2}
 FLAG_SQUARED = 1
 FLAG_TIMING = 2
 NOALLOC_SCRATCH_BYTES = 128 << 20  # KNN_NOALLOC_SCRATCH_BYTES of include/pdist_knn.h
-KERNEL_PATH = {0: "generic-simt", 1: "mfma-fp32", 2: "mfma-bf16", 3: "mfma-fp8", 4: "lowd-simt"}
+KERNEL_PATH = {0: "mfma-16x16x4", 1: "mfma-fp32", 2: "mfma-bf16", 3: "mfma-fp8", 4: "lowd-simt",
+               5: "large-k"}
 ABI_VERSION = 3  # knn_abi_version() of the library this binding matches
 
 # Every symbol include/pdist_knn.h declares.

@@ -86,6 +86,9 @@ struct Plan {
   // group_size split lists are first reduced to n_groups lists per query.
   int group_size = 1, n_groups = 1;
   size_t off_pre_d = 0, off_pre_i = 0;
+  // Large k: queries per chunk and the chunk's distance rows / selections.
+  int64_t bigk_chunk = 0;
+  size_t off_bigk_d = 0, off_bigk_sel_d = 0, off_bigk_sel_i = 0;
   int64_t rows_per_split = 0, n_qblocks = 0;
   size_t elem = 4;
   // Sampled threshold (MFMA paths): a first launch over sample_regions row
@@ -109,7 +112,10 @@ struct Plan {
 };
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
-              kPathLowd = 4;
+              kPathLowd = 4, kPathBigK = 5;
+// Large-k path: dense FP64 distance rows for at most this many bytes of
+// queries at a time (and at most kBigKSelBytes of selected entries).
+constexpr size_t kBigKRowBytes = (size_t)2 << 30, kBigKSelBytes = (size_t)512 << 20;
 
 // Lists one merge workgroup takes (its register staging holds kMaxMerge
 // entries), and the most splits the two-level merge can take.
@@ -291,12 +297,36 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     kc = 4 * k;
   }
   kc = std::min<int64_t>(kc, n_train);
-  if (kc > knn::kMaxCandidates) {
-    if (k > knn::kMaxCandidates)
-      return fail(KNN_EUNSUPPORTED, "k above " + std::to_string(knn::kMaxCandidates) +
-                                         " is not supported");
-    kc = knn::kMaxCandidates;
+  if (k > knn::kMaxCandidates) {
+    // More neighbours than one merge workgroup sorts: exact FP64 distance
+    // rows of a chunk of queries, radix select and rank sort per query.
+    p = Plan();
+    p.n_train = n_train;
+    p.n_test = n_test;
+    p.dim = dim;
+    p.k = k;
+    p.elem = elem;
+    p.kc = (int)k;
+    p.path = kPathBigK;
+    const int64_t nq1 = std::max<int64_t>(n_test, 1);
+    int64_t chunk = std::max<int64_t>(1, (int64_t)(kBigKRowBytes / ((size_t)n_train * 8)));
+    chunk = std::min<int64_t>(chunk, (int64_t)(kBigKSelBytes / ((size_t)k * 12)));
+    p.bigk_chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, nq1));
+    p.n_splits = (int)knn::ceil_div(nq1, p.bigk_chunk);
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+      const size_t at = o;
+      o = align_up(o + bytes);
+      return at;
+    };
+    p.off_scalars = take(sizeof(knn::DeviceScalars));
+    p.off_bigk_d = take((size_t)p.bigk_chunk * n_train * sizeof(double));
+    p.off_bigk_sel_d = take((size_t)p.bigk_chunk * k * sizeof(double));
+    p.off_bigk_sel_i = take((size_t)p.bigk_chunk * k * sizeof(int));
+    p.total = o;
+    return KNN_OK;
   }
+  if (kc > knn::kMaxCandidates) kc = knn::kMaxCandidates;
   p = Plan();
   p.n_train = n_train;
   p.n_test = n_test;
@@ -309,10 +339,13 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   const bool mfma_cap_ok = mfma_cap <= knn::kMfmaMaxCap;
   const int lowd_cap = knn::next_pow2((int)kc + knn::kLowdRowStep);
   p.path = kPathGeneric;
+  if (precision != KNN_PRECISION_EXACT && (dim > 256 || !mfma_cap_ok)) {
+    // Beyond the bf16/fp8 kernels (d <= 256, k' <= kMfmaMaxCap - kMfmaCapSlack):
+    // the exact search, whose answer the low-precision one only reproduces.
+    return make_plan(p, elem, n_train, n_test, dim, k, KNN_PRECISION_EXACT,
+                     oversample > knn::kMaxCandidates ? 0 : oversample, force_generic);
+  }
   if (precision != KNN_PRECISION_EXACT) {
-    if (dim > 256 || !mfma_cap_ok)
-      return fail(KNN_EUNSUPPORTED, "bf16/fp8 search supports dim <= 256 and k' <= " +
-                                        std::to_string(knn::kMfmaMaxCap - knn::kMfmaCapSlack));
     p.path = precision == KNN_PRECISION_BF16 ? kPathMfmaBF16 : kPathMfmaFP8;
     p.dm = std::max(p.path == kPathMfmaBF16 ? 32 : 64, knn::next_pow2((int)dim));
   } else if (!force_generic && elem == 4 && dim >= 16 && dim <= 256 &&
@@ -527,6 +560,54 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   auto* tau = reinterpret_cast<unsigned long long*>(ws + p.off_tau);
   StageTimer timer(stats && (flags & KNN_FLAG_TIMING), stream);
   timer.mark(0);
+  // Launch check, stats read-back, workspace release: the end of every path.
+  auto finish = [&]() -> int {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      if (owned) (void)hipFreeAsync(ws, stream);
+      return fail(KNN_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    }
+    if (stats) {
+      knn::DeviceScalars host_sc;
+      e = hipMemcpyAsync(&host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) {
+        if (owned) (void)hipFreeAsync(ws, stream);
+        return fail(KNN_EHIP, std::string("stats copy: ") + hipGetErrorString(e));
+      }
+      stats->n_uncertified = host_sc.n_uncertified;
+      stats->n_rescued = host_sc.n_rescued;
+      stats->n_widened = host_sc.n_widened;
+      if (timer.on) {
+        stats->prep_ms = timer.elapsed(0, 1);
+        stats->search_ms = timer.elapsed(1, 2);
+        stats->merge_ms = timer.elapsed(2, 3);
+      }
+    }
+    if (owned) {
+      e = hipFreeAsync(ws, stream);
+      if (e != hipSuccess) return fail(KNN_EHIP, std::string("hipFreeAsync: ") + hipGetErrorString(e));
+    }
+    return KNN_OK;
+  };
+
+  if (p.path == kPathBigK) {
+    knn::launch_fill_keys<float>(reinterpret_cast<unsigned long long*>(sc),
+                                 sizeof(knn::DeviceScalars) / 8, 0ull, stream);
+    timer.mark(1);
+    double* D = reinterpret_cast<double*>(ws + p.off_bigk_d);
+    double* sel_d = reinterpret_cast<double*>(ws + p.off_bigk_sel_d);
+    int* sel_i = reinterpret_cast<int*>(ws + p.off_bigk_sel_i);
+    for (int64_t q0 = 0; q0 < n_test; q0 += p.bigk_chunk) {
+      const int64_t nq = std::min<int64_t>(p.bigk_chunk, n_test - q0);
+      knn::launch_sqdist_f64<T>(X_test + q0 * dim, nq, X_train, n_train, (int)dim, D, stream);
+      knn::launch_bigk_select_sort<T>(D, nq, n_train, (int)k, (flags & KNN_FLAG_SQUARED) ? 1 : 0,
+                                      sel_d, sel_i, dist_out + q0 * k, idx_out + q0 * k, stream);
+    }
+    timer.mark(2);
+    timer.mark(3);
+    return finish();
+  }
 
   // 1. fill: scalars are 4 ordered-key slots (max keys start at key(-inf)).
   knn::launch_fill_keys<float>(reinterpret_cast<unsigned long long*>(sc),
@@ -853,33 +934,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   if (stats) knn::launch_count_uncertified(ma.uncert, n_test, sc, stream);
   timer.mark(3);
 
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    if (owned) (void)hipFreeAsync(ws, stream);
-    return fail(KNN_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
-  }
-  if (stats) {
-    knn::DeviceScalars host_sc;
-    e = hipMemcpyAsync(&host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    if (e != hipSuccess) {
-      if (owned) (void)hipFreeAsync(ws, stream);
-      return fail(KNN_EHIP, std::string("stats copy: ") + hipGetErrorString(e));
-    }
-    stats->n_uncertified = host_sc.n_uncertified;
-    stats->n_rescued = host_sc.n_rescued;
-    stats->n_widened = host_sc.n_widened;
-    if (timer.on) {
-      stats->prep_ms = timer.elapsed(0, 1);
-      stats->search_ms = timer.elapsed(1, 2);
-      stats->merge_ms = timer.elapsed(2, 3);
-    }
-  }
-  if (owned) {
-    e = hipFreeAsync(ws, stream);
-    if (e != hipSuccess) return fail(KNN_EHIP, std::string("hipFreeAsync: ") + hipGetErrorString(e));
-  }
-  return KNN_OK;
+  return finish();
 }
 
 template <class T>

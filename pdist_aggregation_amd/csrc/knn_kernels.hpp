@@ -320,6 +320,22 @@ template <class T>
 void launch_sqdist(const T* X, int64_t n_q, const T* Y, int64_t n_t, int d, T* out,
                    hipStream_t s);
 
+// The same in FP64 (differences and sums in double whatever T is).
+template <class T>
+void launch_sqdist_f64(const T* X, int64_t n_q, const T* Y, int64_t n_t, int d, double* out,
+                       hipStream_t s);
+
+// Large k (knn_bigk.hip): for the n_q queries of one chunk, D holds their
+// exact squared distances to all n_t train rows (row-major).  Per query the k
+// smallest (distance, index) keys are found by a radix select (ties at the
+// k-th distance resolved by the lower index), then ordered by a rank sort
+// into dist_out / idx_out rows q0 .. q0 + n_q - 1 (sqrt unless squared).
+// sel_d / sel_i: n_q * k scratch.
+template <class T>
+void launch_bigk_select_sort(const double* D, int64_t n_q, int64_t n_t, int k, int squared,
+                             double* sel_d, int* sel_i, T* dist_out, int64_t* idx_out,
+                             hipStream_t s);
+
 size_t merge_lds_bytes(int n_splits, int kc, size_t elem);
 
 }  // namespace knn

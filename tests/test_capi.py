@@ -42,6 +42,13 @@ def test_workspace_size():
     assert ws(4, 10, 3, 4, 0, 0, 0) == 0  # k == 0
     assert ws(3, 10, 3, 4, 1, 0, 0) == 0  # bad dtype
     assert ws(8, 100, 3, 4, 1, 1, 0) == 0  # f64 + bf16
+    # k above the merge limit (1,984): the large-k path's dense rows + selections
+    big = ws(4, 10_000, 300, 32, 5000, 0, 0)
+    assert big >= 300 * 10_000 * 8 + 300 * 5000 * 12
+    assert ws(8, 10_000, 300, 32, 10_000, 0, 0) > 0  # k = n_train
+    # bf16 / fp8 beyond their kernels (k' > 478, d > 256) size the exact search
+    assert ws(4, 6000, 100, 64, 600, 1, 0) == ws(4, 6000, 100, 64, 600, 0, 0)
+    assert ws(4, 6000, 100, 300, 10, 2, 0) == ws(4, 6000, 100, 300, 10, 0, 0)
 
 
 def _call(**over):
