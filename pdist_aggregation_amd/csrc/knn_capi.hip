@@ -113,6 +113,10 @@ struct Plan {
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
               kPathLowd = 4, kPathBigK = 5;
+// Float32 with d % 4 != 0 takes the 32 x 32 MFMA kernel through zero-padded
+// copies of the rows only while the train copy stays within this size; larger
+// train sets use the 16 x 16 x 4 MFMA kernel on the caller's rows (no copy).
+constexpr size_t kMaxPadBytes = (size_t)1 << 30;
 // Large-k path: dense FP64 distance rows for at most this many bytes of
 // queries at a time (and at most kBigKSelBytes of selected entries).
 constexpr size_t kBigKRowBytes = (size_t)2 << 30, kBigKSelBytes = (size_t)512 << 20;
@@ -348,8 +352,8 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   if (precision != KNN_PRECISION_EXACT) {
     p.path = precision == KNN_PRECISION_BF16 ? kPathMfmaBF16 : kPathMfmaFP8;
     p.dm = std::max(p.path == kPathMfmaBF16 ? 32 : 64, knn::next_pow2((int)dim));
-  } else if (!force_generic && elem == 4 && dim >= 16 && dim <= 256 &&
-             mfma_cap_ok) {
+  } else if (!force_generic && elem == 4 && dim >= 16 && dim <= 256 && mfma_cap_ok &&
+             (dim % 4 == 0 || (size_t)n_train * ((dim + 3) & ~(int64_t)3) * 4 <= kMaxPadBytes)) {
     p.path = kPathMfmaF32;
     // 192 is a search width too (48 chunks of 16 bytes per row): d in
     // (128, 192] would otherwise pad to 256 and spend up to half the MFMAs on zeros.

@@ -53,7 +53,9 @@ CASES = [("exact", 64, 10), ("exact", 128, 10), ("exact", 32, 1), ("bf16", 128, 
          # float32 search widths 192 and 256 (one or two workgroups per CU)
          ("exact", 160, 10), ("exact", 256, 10),
          # low-d kernel (one lane per query, padded to 4 / 8 / 16 features)
-         ("exact", 3, 10), ("exact", 7, 16), ("exact", 12, 5)]
+         ("exact", 3, 10), ("exact", 7, 16), ("exact", 12, 5),
+         # float32 MFMA through zero-padded row copies (d % 4 != 0)
+         ("exact", 17, 10), ("exact", 50, 10), ("exact", 130, 10), ("exact", 255, 16)]
 
 
 @pytest.mark.parametrize("precision,d,k", CASES)

@@ -27,6 +27,7 @@ CASES = [
     ("bf16", 60000, 300, 128, 32, 100),    # k'=128: 16 lists -> groups of 7
     ("fp8", 60000, 200, 128, 10, 120),     # k'=40: 51 lists -> groups of 3
     ("exact", 40000, 100, 32, 50, 17),     # below the limit: one level (control)
+    ("exact", 60000, 300, 50, 20, 150),    # zero-padded rows (d % 4 != 0), k'=28: groups of 3
 ]
 
 
@@ -41,7 +42,7 @@ def test_forced_splits_match_oracle(require_gpu, monkeypatch, precision, nt, nq,
     (dist, idx), _, st = pa.parallel_knn(Y, X, k, return_distance=True, squared=True,
                                          precision=precision, return_stats=True)
     _say(f"    stats {st}")
-    two_level = splits != 17
+    two_level = splits not in (17,)
     assert (st["n_splits"] * st["k_candidates"] > 2048) == two_level, st
     assert_sorted_unique(idx, dist, nt, X, Y)
     if precision == "exact":
