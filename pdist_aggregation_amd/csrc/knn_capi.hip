@@ -232,6 +232,10 @@ constexpr int kGuessMaxJ = 128;
 // 100-350 rows for FP8 there, and about 2,000 (a whole Gaussian) for FP8 on
 // the 1,000-Gaussian mixture in d = 256 with k' = 400 (capacity 4096).
 constexpr int kCollMin = 1024, kCollMax = 16384;
+// Collect launch geometry: per-lane list capacity (two halves per query) and
+// the shortest split it is given.
+constexpr int kCollectCap = 64;
+constexpr int64_t kCollectMinRows = 4096;
 
 // runs: sampled runs per (query, region) the kernel writes (2 for the MFMA
 // kernel's two lane halves, 1 for the low-d kernel's one lane per query).
@@ -910,6 +914,26 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       ca.coll_cnt = reinterpret_cast<int*>(ws + p.off_coll_cnt);
       ca.coll_idx = reinterpret_cast<int*>(ws + p.off_coll_idx);
       ca.coll_cap = p.coll_cap;
+      // The collect launch keeps no per-split lists, so it takes shorter
+      // splits (its latency is one workgroup's sweep: with a few widened
+      // queries, that sweep is all it costs): lane lists of kCollectCap
+      // indices, flushed when half full, in the search's candidate scratch
+      // (values and indices, idle by now), as many splits as that holds, up
+      // to splits of kCollectMinRows rows.
+      ca.cap = kCollectCap;
+      ca.cand_i = reinterpret_cast<int*>(ws + p.off_cand_d);
+      ca.cand_d = reinterpret_cast<float*>(ca.cand_i);  // collect lists hold indices only
+      {
+        const int bq = knn::mfma_block_queries(fa.fmt, p.dm);
+        const int64_t room = (int64_t)((p.off_part_d - p.off_cand_d) / sizeof(int));
+        const int64_t fit = room / ((int64_t)p.n_qblocks * bq * kCollectCap);
+        const int64_t want = knn::ceil_div(n_train, kCollectMinRows);
+        const int64_t splits_c = std::max<int64_t>(p.n_splits, std::min(fit, want));
+        ca.rows_per_split = knn::ceil_div(knn::ceil_div(n_train, splits_c), knn::kMfmaRowStep) *
+                            knn::kMfmaRowStep;
+        ca.split_stride = ca.rows_per_split;
+        ca.n_splits = (int)knn::ceil_div(n_train, ca.rows_per_split);
+      }
       knn::launch_fill_u32(reinterpret_cast<unsigned*>(ca.coll_cnt), n_test, 0u, stream);
       knn::launch_mfma_collect(ca, stream);
       knn::CollectArgs<T> cr;
