@@ -9,6 +9,9 @@ in device memory:
 * ``plain-f64``: X 1,000 x 50 and Y 10,000 x 50 float64 U[0,1) from
   numpy RandomState(0) (Y drawn first), k = 10 -- the reference's
   CPU-runnable case, the inputs of tests/golden/make_golden.py;
+* ``grid-f64-d500``: Y 100,000 x 500 and X 1,000 x 500 float64 U[0,1) from
+  RandomState(1), k = 10 -- the largest float64 shape of the reference's
+  benchmark grid, on the 16 x 16 x 4 float64 MFMA kernel;
 * ``lowd``: 65,536 x 65,536 points on the unit sphere with N(0, 0.01^2)
   radial jitter, d = 3, k = 16, float32; forward, then forward + backward
   through ``knn_distances``;
@@ -57,6 +60,13 @@ def _inputs(name):
         rs = np.random.RandomState(0)
         Y = rs.rand(10_000, 50)
         X = rs.rand(1_000, 50)
+        return Y, X, 10, "exact", 0
+    if name == "grid-f64-d500":
+        # the largest float64 shape of the reference benchmark grid
+        # (benchmarks/config.yml of the reference: 100,000 train x 1,000 test x 500)
+        rs = np.random.RandomState(1)
+        Y = rs.rand(100_000, 500)
+        X = rs.rand(1_000, 500)
         return Y, X, 10, "exact", 0
     if name == "lowd":
         rs = np.random.RandomState(0)
@@ -110,6 +120,13 @@ def run(name, reps=3):
                   f"path {st.kernel_path}, splits {st.n_splits}, k' {st.k_candidates}, "
                   f"uncertified {st.n_uncertified}, re-searched {st.n_rescued}", flush=True)
     del ws
+    if name == "grid-f64-d500":
+        # the largest float64 shape of the reference benchmark grid
+        # (benchmarks/config.yml of the reference: 100,000 train x 1,000 test x 500)
+        rs = np.random.RandomState(1)
+        Y = rs.rand(100_000, 500)
+        X = rs.rand(1_000, 500)
+        return Y, X, 10, "exact", 0
     if name == "lowd":
         from .autograd import knn_distances
 
@@ -134,7 +151,8 @@ def run(name, reps=3):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("names", nargs="*",
-                    default=["plain-f64", "lowd", "bf16-mixture", "fp8-mixture"])
+                    default=["plain-f64", "grid-f64-d500", "lowd", "bf16-mixture",
+                             "fp8-mixture"])
     ap.add_argument("--reps", type=int, default=3)
     args = ap.parse_args(argv)
     for name in args.names:
