@@ -263,13 +263,18 @@ def main():
     # Host-to-device copy of both sets the way parallel_knn does it for numpy
     # inputs (8 MiB chunks through pinned buffers); timed once, reported apart
     # from the device-resident throughput.
+    # The first upload of a process also pays for the first touch of the
+    # pinned staging buffers; the second is what a later call sees.
     stream = torch.cuda.current_stream(dev)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    Yd = knn_mod._upload(torch.from_numpy(Y), torch, dev, stream)
-    Xd = knn_mod._upload(torch.from_numpy(X), torch, dev, stream)
-    torch.cuda.synchronize()
-    h2d_ms = 1e3 * (time.perf_counter() - t0)
+    h2d = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Yd = knn_mod._upload(torch.from_numpy(Y), torch, dev, stream)
+        Xd = knn_mod._upload(torch.from_numpy(X), torch, dev, stream)
+        torch.cuda.synchronize()
+        h2d.append(1e3 * (time.perf_counter() - t0))
+    h2d_first_ms, h2d_ms = h2d
     flops = 2.0 * args.nq * args.nt * args.dim
     print(f"[bench] inputs ready: X {X.shape} Y {Y.shape}, H2D {h2d_ms:.1f} ms",
           file=sys.stderr, flush=True)
@@ -367,8 +372,11 @@ def main():
                               "neighbours); tests/test_gpu_parity.py checks the same 1e4 x 1e6 "
                               "x 128 results against the CPU oracle",
         "h2d_ms": h2d_ms,
+        "h2d_first_ms": h2d_first_ms,
+        "h2d_GBps": (Y.nbytes + X.nbytes) / (h2d_ms * 1e-3) / 1e9,
         "h2d_note": "host-to-device copy of X and Y through parallel_knn's pinned chunked "
-                    "upload, not part of value (inputs resident in HBM)",
+                    "upload (second copy of the process; h2d_first_ms: the first), not part of "
+                    "value (inputs resident in HBM)",
         "queries_per_s_with_h2d": args.nq / ((h["ms_per_step"] + h2d_ms) * 1e-3),
         "host": host_info(threads),
         "results": results,
