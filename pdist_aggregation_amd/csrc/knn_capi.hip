@@ -623,6 +623,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
                                0ull, stream);
   knn::launch_fill_keys<float>(tau, n_test, knn::ordered_key(__builtin_huge_val()), stream);
   knn::MergeArgs<T> ma;
+  ma.unsorted = 0;
   ma.pre_d = nullptr;
   ma.pre_i = nullptr;
   ma.group_size = 1;
@@ -844,6 +845,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     timer.mark(2);
     ma.part_d = reinterpret_cast<const T*>(fa.part_d);
     ma.part_i = fa.part_i;
+    ma.unsorted = 1;  // the MFMA search copies short split lists unsorted
   }
   // 4. merge + exact re-rank
   ma.X = X_test;
@@ -874,6 +876,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       m.part_d = pm.pre_d;
       m.part_i = pm.pre_i;
       m.n_splits = p.n_groups;
+      m.unsorted = 0;  // the pre-merge writes its group lists sorted
     }
     knn::launch_merge_rerank<T>(m, stream);
   };

@@ -216,6 +216,10 @@ struct MergeArgs {
   const T* part_d;
   const int* part_i;
   int n_splits, kc, k;
+  // 1: the lists at part_d / part_i are not sorted (the MFMA search copies a
+  // split's list as it stands unless it outgrew k'); every entry at or below
+  // the query's threshold is then ranked.  0: each list is ascending.
+  int unsorted;
   int squared;  // output squared distances instead of Euclidean
   // Certification: the approximate values were computed on vectors xq, yq
   // with |x - xq| <= xerr[i] (nullptr: 0), |y - yq| <= max_yerr, rows norms

@@ -224,6 +224,68 @@ __device__ inline int count_less(const T* kd, const int* ki, int n, T x, int xi)
   return r;
 }
 
+// The kth-smallest (1-based) of the float values v[i] held by the workgroup's
+// threads (entries with id[i] >= 0, rounds i < rounds), by a radix select
+// over their order-preserving 32-bit keys: four passes of 8 bits, each an LDS
+// histogram of the entries that match the digits chosen so far and one
+// wave's prefix scan that picks the next digit.  The caller guarantees at
+// least kth such entries.  Every thread calls it (it holds barriers).
+template <int NT, int N>
+__device__ float block_select_kth(const float (&v)[N], const int (&id)[N], int rounds, int kth,
+                                  unsigned* hist) {
+  __shared__ unsigned pick_s[2];  // digits chosen so far, rank left within them
+  const int tid = threadIdx.x, lane = tid & 63;
+  unsigned key[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const unsigned b = __float_as_uint(v[i]);
+    key[i] = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  }
+  unsigned prefix = 0, mask = 0, rank = (unsigned)kth;
+#pragma unroll
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 24 - 8 * pass;
+    for (int b = tid; b < 256; b += NT) hist[b] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (i < rounds && id[i] >= 0 && (key[i] & mask) == prefix)
+        atomicAdd(&hist[(key[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    if (tid < 64) {
+      // lane l holds bins 4l .. 4l+3; inclusive scan of the lane sums
+      unsigned c[4], s = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        c[j] = hist[4 * lane + j];
+        s += c[j];
+      }
+      unsigned incl = s;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+      }
+      const unsigned before = incl - s;
+      // the lane whose bins reach `rank` picks the digit
+      if (before < rank && incl >= rank) {
+        unsigned acc = before;
+        int j = 0;
+        while (acc + c[j] < rank) acc += c[j++];
+        pick_s[0] = prefix | ((unsigned)(4 * lane + j) << shift);
+        pick_s[1] = rank - acc;
+      }
+    }
+    __syncthreads();
+    prefix = pick_s[0];
+    rank = pick_s[1];
+    mask |= 255u << shift;
+    __syncthreads();  // pick_s is rewritten by the next pass
+  }
+  const unsigned b = (prefix & 0x80000000u) ? (prefix & 0x7fffffffu) : ~prefix;
+  return __uint_as_float(b);
+}
+
 // One workgroup of NT threads per query (final merge) or per (query, group of
 // splits) (pre-merge, a.pre_d set); the launch guarantees lists * k' <= NT *
 // kPerThread.
@@ -260,11 +322,12 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
   // or every entry passed the sampled guess): entries above it are dropped.
   // Without k' entries at or below it the query fails the check below.
   const T tq = a.tau ? (T)from_ordered_key(a.tau[q]) : pos_inf<T>();
-  // Each split's list is sorted, so the k'-th smallest of the first r
+  // When each split's list is sorted, the k'-th smallest of the first r
   // entries of every split (r * n_splits >= k') bounds the k'-th value from
   // above: entries past it are dropped too, which leaves a few more than k'.
   const int r0 = (a.kc + lists - 1) / lists + 1;
   const int r = a.kc < r0 ? a.kc : r0;
+  const bool sorted = !a.unsorted;
   T v[kPerThread];
   int id[kPerThread];
   if (tid == 0) n_s = 0;
@@ -278,7 +341,7 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
     if (!(id[i] >= 0 && id[i] < a.n_t && v[i] <= tq)) id[i] = -1;  // never index past Y
   }
   const int rounds = (total + NT - 1) / NT;  // uniform
-  {
+  if (sorted) {
     bool head[kPerThread];
     int at[kPerThread];
 #pragma unroll
@@ -295,10 +358,32 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
       }
   }
   __syncthreads();
-  const int nh = n_s;
+  const int nh = sorted ? n_s : 0;  // unsorted lists: no head cut
   if (tid == 0) cut_s = tq;
   __syncthreads();
-  if (nh >= a.kc) {
+  if constexpr (sizeof(T) == 4) {
+    // Unsorted lists: the k'-th smallest value of all entries at or below
+    // the threshold, selected exactly (every barrier is uniform: a.unsorted
+    // is a launch argument and n_s a shared value).
+    if (!sorted) {
+      if (tid == 0) n_s = 0;
+      __syncthreads();
+      int cnt = 0;
+#pragma unroll
+      for (int i = 0; i < kPerThread; ++i) cnt += (i < rounds && id[i] >= 0) ? 1 : 0;
+      if (cnt) atomicAdd(&n_s, cnt);
+      __syncthreads();
+      const int nv = n_s;
+      __syncthreads();
+      if (nv >= a.kc) {
+        __shared__ unsigned hist[256];
+        const float c = block_select_kth<NT>(v, id, rounds, a.kc, hist);
+        if (tid == 0) cut_s = c;
+        __syncthreads();
+      }
+    }
+  }
+  if (sorted && nh >= a.kc) {
     for (int e = tid; e < nh; e += NT) {
       const T x = sd[e];
       const int xi = si[e];
