@@ -450,8 +450,13 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   size_t ncand = (size_t)p.n_qblocks * p.n_splits * bq * p.cap;
   if (p.guess && p.path == kPathGeneric)  // the sampling launch's lists share this scratch
     ncand = std::max(ncand, (size_t)p.n_qblocks * p.sample_regions * bq * kGenericSampleCap);
-  p.off_cand_d = take(ncand * celem);
-  p.off_cand_i = take(ncand * sizeof(int));
+  if (mfma) {  // (value, row) pairs; the collect launch reuses the space for indices
+    p.off_cand_d = take(ncand * 2 * sizeof(float));
+    p.off_cand_i = p.off_cand_d;
+  } else {
+    p.off_cand_d = take(ncand * celem);
+    p.off_cand_i = take(ncand * sizeof(int));
+  }
   const size_t npart = (size_t)nq1 * p.n_splits * p.kc;
   p.off_part_d = take(npart * celem);
   p.off_part_i = take(npart * sizeof(int));

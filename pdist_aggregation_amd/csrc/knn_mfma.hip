@@ -209,7 +209,10 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   const int64_t ybeg = (int64_t)split * a.split_stride;
   const int64_t yend = ybeg + a.rows_per_split < a.n_t ? ybeg + a.rows_per_split : a.n_t;
   const size_t blk = (size_t)split * a.n_qblocks + qb;
-  float* cand_d = a.cand_d + blk * BQ * a.cap;
+  // Candidate lists of the search launches: (value bits, row) pairs, one
+  // 8-byte store per survivor (two 4-byte stores cost about 0.13 ms more at
+  // 1e4 x 1e6 x 128).  The collect launch keeps row indices only, in cand_i.
+  uint2* cand = reinterpret_cast<uint2*>(a.cand_d) + blk * BQ * a.cap;
   int* cand_i = a.cand_i + blk * BQ * a.cap;
   float* sd = reinterpret_cast<float*>(dyn_smem) + (size_t)w * 2 * a.cap;
   int* si = reinterpret_cast<int*>(sd + a.cap);
@@ -411,15 +414,15 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   // ---- list maintenance (wave-local) ----------------------------------------
   // Sort the two halves of query ql together, keep the best min(n, kc).
   auto sort_query = [&](int ql, int n0, int n1) {
-    float* cd = cand_d + (size_t)ql * a.cap;
-    int* ci = cand_i + (size_t)ql * a.cap;
+    const uint2* cp = cand + (size_t)ql * a.cap;
     const int n = n0 + n1;
     const int P = next_pow2(n);
     for (int e = lane; e < P; e += 64) {
       const bool ok = e < n;
       const int src = e < n0 ? e : half + (e - n0);
-      sd[e] = ok ? cd[src] : __builtin_huge_valf();
-      si[e] = ok ? ci[src] : INT_MAX;
+      const uint2 pr = ok ? cp[src] : make_uint2(0x7f800000u, (unsigned)INT_MAX);
+      sd[e] = __uint_as_float(pr.x);
+      si[e] = (int)pr.y;
     }
     wave_sync();
     for (int size = 2; size <= P; size <<= 1) {
@@ -455,12 +458,10 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
       sort_query(ql, n0, n1);
       const int m = n0 + n1 < a.kc ? n0 + n1 : a.kc;
       const int m0 = (m + 1) >> 1;
-      float* cd = cand_d + (size_t)ql * a.cap;
-      int* ci = cand_i + (size_t)ql * a.cap;
+      uint2* cp = cand + (size_t)ql * a.cap;
       for (int e = lane; e < m; e += 64) {
         const int dst = e < m0 ? e : half + (e - m0);
-        cd[dst] = sd[e];
-        ci[dst] = si[e];
+        cp[dst] = make_uint2(__float_as_uint(sd[e]), (unsigned)si[e]);
       }
       if (lane == 0) {
         cnt_s[2 * ql] = m0;
@@ -524,11 +525,10 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   };
   // Appends go through buffer stores on this workgroup's candidate block, so
   // the address of an append is one 32-bit offset.
-  const int cand_bytes = BQ * a.cap * 4;
-  const __amdgpu_buffer_rsrc_t cdrs =
-      __builtin_amdgcn_make_buffer_rsrc(cand_d, (short)0, cand_bytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t cprs =
+      __builtin_amdgcn_make_buffer_rsrc(cand, (short)0, BQ * a.cap * 8, 0x00020000);
   const __amdgpu_buffer_rsrc_t cirs =
-      __builtin_amdgcn_make_buffer_rsrc(cand_i, (short)0, cand_bytes, 0x00020000);
+      __builtin_amdgcn_make_buffer_rsrc(cand_i, (short)0, BQ * a.cap * 4, 0x00020000);
   // Selection of one finished tile.  Lane values come in 4 quads of 4
   // consecutive train rows; the common case costs a min-tree and one
   // compare, a tile with survivors tests the 4 quad minima and then only the
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
         // offset past the buffer, which the range check drops (no divergent
         // branch).
         const int ql = wq0 + g * 32 + c;
-        const int base = (ql * a.cap + h * half) * 4;
+        const int base = ql * a.cap + h * half;  // entries
         const int yb = (int)y0 + h * 4;
         int n = cnt[g];
         uint64_t qm[4];
@@ -570,12 +570,16 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
             if (vm[e] == 0) continue;
             const float v = acc[g][4 * r4 + e];
             const bool hit = v <= tau[g];
-            const int off = hit ? base + n * 4 : kOOB;
-            if constexpr (MODE != kModeCollect)  // a collect list keeps indices only
-              __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), cdrs, off, 0, 0);
-            __builtin_amdgcn_raw_buffer_store_b32(yb + r4 * 8 + e, cirs, off, 0, 0);
+            if constexpr (MODE == kModeCollect) {  // a collect list keeps indices only
+              __builtin_amdgcn_raw_buffer_store_b32(yb + r4 * 8 + e, cirs,
+                                                    hit ? (base + n) * 4 : kOOB, 0, 0);
+            } else {
+              typedef int i32x2 __attribute__((ext_vector_type(2)));
+              const i32x2 pr = {(int)__float_as_uint(v), yb + r4 * 8 + e};
+              __builtin_amdgcn_raw_buffer_store_b64(pr, cprs, hit ? (base + n) * 8 : kOOB, 0, 0);
+            }
             n += hit;
-            n_stores += MODE == kModeCollect ? 1 : 2;
+            n_stores += 1;
           }
         }
         cnt[g] = n;
@@ -663,7 +667,20 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     load_tile(1, B1());
     // At step t's wait: n_stores when steps t-2 and t-1 began.
     int st_m2 = 0, st_m1 = 0;
+    // Wave priority rotates with the step, offset by the wave's slot on its
+    // SIMD, so the (up to four) waves sharing a SIMD take turns at issue
+    // priority.  Left to age order, the oldest workgroups run ahead and the
+    // last ones finish alone on their CUs (at 1e4 x 1e6 x 128 the sweeps
+    // spread over 1.7 - 2.2 ms instead of 1.9 - 2.1 ms).
+    unsigned wslot;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID, 0, 4)" : "=s"(wslot));
     auto step = [&](int t, auto cur, auto ahead) {
+      switch ((t + (int)wslot) & 3) {
+        case 0: __builtin_amdgcn_s_setprio(0); break;
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        default: __builtin_amdgcn_s_setprio(3); break;
+      }
       // Tile t (loaded two steps ago) complete and visible; `ahead`, read in
       // step t-1, free for tile t+2.
       wait_barrier_upto<kMinLoads>(kMinLoads + n_stores - st_m2,
@@ -770,8 +787,9 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
       dst[u] = mine ? ((int64_t)q * a.n_splits + split) * a.kc + j : -1;
       const bool has = mine && j < n0 + n1;
       const int src = ql * a.cap + (j < n0 ? j : half + (j - n0));
-      vd[u] = has ? cand_d[src] : __builtin_huge_valf();
-      vi[u] = has ? cand_i[src] : -1;
+      const uint2 pr = has ? cand[src] : make_uint2(0x7f800000u, 0xffffffffu);
+      vd[u] = __uint_as_float(pr.x);
+      vi[u] = (int)pr.y;
     }
 #pragma unroll
     for (int u = 0; u < kCopyBatch; ++u)
