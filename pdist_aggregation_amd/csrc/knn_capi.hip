@@ -108,7 +108,7 @@ struct Plan {
          off_xerr = 0, off_xscale = 0, off_yscale = 0, off_slots = 0, off_sample = 0,
          off_guess = 0, off_failed = 0, off_failed_qb = 0, off_uncert = 0, off_xpad = 0,
          off_ypad = 0, off_widen = 0, off_widen_qb = 0, off_ubound = 0, off_coll_cnt = 0,
-         off_coll_idx = 0, total = 0;
+         off_coll_idx = 0, off_part_n = 0, total = 0;
 };
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
@@ -460,6 +460,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
   const size_t npart = (size_t)nq1 * p.n_splits * p.kc;
   p.off_part_d = take(npart * celem);
   p.off_part_i = take(npart * sizeof(int));
+  if (mfma) p.off_part_n = take((size_t)nq1 * p.n_splits * sizeof(int));
   if (p.group_size > 1) {
     const size_t npre = (size_t)nq1 * p.n_groups * p.kc;
     p.off_pre_d = take(npre * celem);
@@ -629,6 +630,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   knn::launch_fill_keys<float>(tau, n_test, knn::ordered_key(__builtin_huge_val()), stream);
   knn::MergeArgs<T> ma;
   ma.unsorted = 0;
+  ma.part_n = nullptr;
   ma.pre_d = nullptr;
   ma.pre_i = nullptr;
   ma.group_size = 1;
@@ -745,6 +747,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     fa.tau = tau;
     fa.part_d = reinterpret_cast<float*>(ws + p.off_part_d);
     fa.part_i = reinterpret_cast<int*>(ws + p.off_part_i);
+    fa.part_n = reinterpret_cast<int*>(ws + p.off_part_n);
     fa.ynorm = reinterpret_cast<float*>(ynorm);
     // Shared-threshold slots (+inf = nothing published), used when at least
     // kSlotJ splits exist and no sampled threshold is in use: with one, lists
@@ -851,6 +854,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     ma.part_d = reinterpret_cast<const T*>(fa.part_d);
     ma.part_i = fa.part_i;
     ma.unsorted = 1;  // the MFMA search copies short split lists unsorted
+    ma.part_n = fa.part_n;  // and writes only their entries
   }
   // 4. merge + exact re-rank
   ma.X = X_test;
@@ -881,7 +885,8 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       m.part_d = pm.pre_d;
       m.part_i = pm.pre_i;
       m.n_splits = p.n_groups;
-      m.unsorted = 0;  // the pre-merge writes its group lists sorted
+      m.unsorted = 0;  // the pre-merge writes its group lists sorted, all k' slots
+      m.part_n = nullptr;
     }
     knn::launch_merge_rerank<T>(m, stream);
   };

@@ -92,6 +92,9 @@ struct MfmaArgs {
   unsigned long long* tau;
   float* part_d;
   int* part_i;
+  // part_n[q * n_splits + s]: entries of (query q, split s) written to the
+  // lists above (the rest of the k' slots are left unwritten).
+  int* part_n;
   // Cross-split threshold sharing: qslots[q * slot_stride + split] (init
   // +inf; slot_stride = n_splits rounded up to 8) receives the split's
   // (slot_rank+1)-th smallest value at each list compaction, with
@@ -220,6 +223,9 @@ struct MergeArgs {
   // split's list as it stands unless it outgrew k'); every entry at or below
   // the query's threshold is then ranked.  0: each list is ascending.
   int unsorted;
+  // Valid entries per (query, list), [n_q][n_splits] (nullptr: all k' slots
+  // are written, the unused ones with +inf / -1).
+  const int* part_n;
   int squared;  // output squared distances instead of Euclidean
   // Certification: the approximate values were computed on vectors xq, yq
   // with |x - xq| <= xerr[i] (nullptr: 0), |y - yq| <= max_yerr, rows norms

@@ -330,12 +330,20 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
   const bool sorted = !a.unsorted;
   T v[kPerThread];
   int id[kPerThread];
+  // List of entry e: (e + 0.5) / k' is at least 0.5 / k' from an integer, far
+  // more than the float product's error for e < kMaxMerge, so truncation is
+  // exact (an integer division per entry cost about 10 % of the merge).
+  const float inv_kc = 1.f / (float)a.kc;
   if (tid == 0) n_s = 0;
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kPerThread; ++i) {
     const int e = tid + i * NT;
-    const bool in = e < total;
+    bool in = e < total;
+    if (in && a.part_n) {  // unwritten slots past the list's entries
+      const int l = (int)(((float)e + 0.5f) * inv_kc);
+      in = e - l * a.kc < a.part_n[(size_t)q * a.n_splits + first + l];
+    }
     id[i] = in ? pi[e] : -1;
     v[i] = in ? pd[e] : pos_inf<T>();
     if (!(id[i] >= 0 && id[i] < a.n_t && v[i] <= tq)) id[i] = -1;  // never index past Y

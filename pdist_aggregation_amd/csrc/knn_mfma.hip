@@ -533,20 +533,29 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   // consecutive train rows; the common case costs a min-tree and one
   // compare, a tile with survivors tests the 4 quad minima and then only the
   // values of quads that hold a survivor.
-  auto select = [&](f32x16(&acc)[QG], int64_t y0, int stp) {
+  auto select = [&](f32x16(&acc)[QG], int64_t y0) {
     bool overflow = false;
+    // IEEE minimum (NaN-propagating, v_minimum3_f32 on gfx950: no operand
+    // canonicalisation).  A NaN makes a quad "not above tau", so its values
+    // are tested one by one and a NaN hides no other row.  One vote covers
+    // every query group of the wave (the common case is no survivor at all).
+    float m4q[QG][4];
+    bool hot[QG], any_hot = false;
 #pragma unroll
     for (int g = 0; g < QG; ++g) {
-      // IEEE minimum (NaN-propagating, v_minimum3_f32 on gfx950: no operand
-      // canonicalisation).  A NaN makes a quad "not above tau", so its
-      // values are tested one by one and a NaN hides no other row.
-      float m4[4];
 #pragma unroll
       for (int r4 = 0; r4 < 4; ++r4)
-        m4[r4] = fmin_ieee(fmin_ieee(acc[g][4 * r4], acc[g][4 * r4 + 1]),
-                           fmin_ieee(acc[g][4 * r4 + 2], acc[g][4 * r4 + 3]));
-      const float m = fmin_ieee(fmin_ieee(m4[0], m4[1]), fmin_ieee(m4[2], m4[3]));
-      if (__any(!(m > tau[g]))) {
+        m4q[g][r4] = fmin_ieee(fmin_ieee(acc[g][4 * r4], acc[g][4 * r4 + 1]),
+                               fmin_ieee(acc[g][4 * r4 + 2], acc[g][4 * r4 + 3]));
+      const float m = fmin_ieee(fmin_ieee(m4q[g][0], m4q[g][1]), fmin_ieee(m4q[g][2], m4q[g][3]));
+      hot[g] = !(m > tau[g]);
+      any_hot |= hot[g];
+    }
+    if (!__any(any_hot)) return;
+#pragma unroll
+    for (int g = 0; g < QG; ++g) {
+      const float* m4 = m4q[g];
+      if (__any(hot[g])) {
         // Wave-wide votes (independent compares into scalar masks) pick the
         // quads, then the values, that hold a survivor in any lane; a value
         // with one is stored by every lane, the lanes without one at an
@@ -592,9 +601,12 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
       else
         compact(thr);
     }
-    if constexpr (MODE == kModeCollect) return;  // fixed threshold
-    // Refresh at steps 3, 7, 15, then every 32 (cheap early sharing, rare later).
-    if ((stp < 32 && ((stp + 1) & stp) == 0 && stp >= 3) || (stp & 31) == 31) {
+  };
+  // Threshold refresh (search launches; the collect launch keeps its fixed
+  // threshold), called by the sweep after ring cycles 1, 2, 5 and then every
+  // 11 (every 33 rows steps: cheap early sharing, rare later).
+  auto refresh = [&]() {
+    if constexpr (MODE == kModeSearch || MODE == kModeResearch) {
       // Thresholds shared across the splits of a query:
       //  * the best k'-th value any split has reached (atomicMin key);
       //  * slot s holds split s's (slot_rank+1)-th smallest value, so the
@@ -658,7 +670,7 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   // instead of waiting for stores issued moments ago.  Only stores counted
   // in n_stores are allowed for; any other load or store issued since only
   // makes the wait longer than needed, never too short.
-  auto sweep = [&](auto&& sel) {
+  auto sweep = [&](auto&& sel, auto&& after_cycle) {
     using B0 = std::integral_constant<int, 0>;
     using B1 = std::integral_constant<int, 1>;
     using B2 = std::integral_constant<int, 2>;
@@ -667,20 +679,14 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     load_tile(1, B1());
     // At step t's wait: n_stores when steps t-2 and t-1 began.
     int st_m2 = 0, st_m1 = 0;
-    // Wave priority rotates with the step, offset by the wave's slot on its
-    // SIMD, so the (up to four) waves sharing a SIMD take turns at issue
-    // priority.  Left to age order, the oldest workgroups run ahead and the
-    // last ones finish alone on their CUs (at 1e4 x 1e6 x 128 the sweeps
-    // spread over 1.7 - 2.2 ms instead of 1.9 - 2.1 ms).
+    // Wave priority rotates with the ring cycle (3 steps), offset by the
+    // wave's slot on its SIMD, so the (up to four) waves sharing a SIMD take
+    // turns at issue priority.  Left to age order, the oldest workgroups run
+    // ahead and the last ones finish alone on their CUs (at 1e4 x 1e6 x 128
+    // the sweeps spread over 1.7 - 2.2 ms instead of 1.9 - 2.1 ms).
     unsigned wslot;
     asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID, 0, 4)" : "=s"(wslot));
     auto step = [&](int t, auto cur, auto ahead) {
-      switch ((t + (int)wslot) & 3) {
-        case 0: __builtin_amdgcn_s_setprio(0); break;
-        case 1: __builtin_amdgcn_s_setprio(1); break;
-        case 2: __builtin_amdgcn_s_setprio(2); break;
-        default: __builtin_amdgcn_s_setprio(3); break;
-      }
       // Tile t (loaded two steps ago) complete and visible; `ahead`, read in
       // step t-1, free for tile t+2.
       wait_barrier_upto<kMinLoads>(kMinLoads + n_stores - st_m2,
@@ -689,12 +695,19 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
       st_m1 = n_stores;
       load_tile(t + 2, ahead);
       compute(acc, tbuf[decltype(cur)::value]);
-      sel(acc, ybeg + (int64_t)t * kBY, t);
+      sel(acc, ybeg + (int64_t)t * kBY);
     };
-    for (int t = 0; t < ntiles; t += kNBuf) {
+    for (int t = 0, cyc = 0; t < ntiles; t += kNBuf, ++cyc) {
+      switch ((cyc + (int)wslot) & 3) {
+        case 0: __builtin_amdgcn_s_setprio(0); break;
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        default: __builtin_amdgcn_s_setprio(3); break;
+      }
       step(t, B0(), B2());
       if (t + 1 < ntiles) step(t + 1, B1(), B0());
       if (t + 2 < ntiles) step(t + 2, B2(), B1());
+      after_cycle(cyc);
     }
     // No load may still be writing LDS when the workgroup ends.
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -709,7 +722,7 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     for (int g = 0; g < QG; ++g)
 #pragma unroll
       for (int i = 0; i < kSampleJ; ++i) best[g][i] = __builtin_huge_valf();
-    sweep([&](f32x16(&acc)[QG], int64_t, int) {
+    sweep([&](f32x16(&acc)[QG], int64_t) {
 #pragma unroll
       for (int g = 0; g < QG; ++g) {
         float m = acc[g][0];  // a NaN value drops this step's minimum (m < b fails)
@@ -722,7 +735,7 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
           best[g][0] = fminf(best[g][0], m);
         }
       }
-    });
+    }, [](int) {});
 #pragma unroll
     for (int g = 0; g < QG; ++g) {
       const int64_t q = q0 + wq0 + g * 32 + c;
@@ -736,7 +749,13 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     return;
   }
 
-  sweep(select);
+  int next_refresh = 1;
+  sweep(select, [&](int cyc) {
+    if (cyc == next_refresh) {
+      refresh();
+      next_refresh = cyc == 1 ? 2 : cyc == 2 ? 5 : cyc + 11;
+    }
+  });
   if constexpr (MODE == kModeCollect) {
     flush(0);  // whatever is left
     return;
@@ -749,7 +768,8 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   // of query qq is item qq * k' + j, items are dealt to the lanes, and each
   // lane has kCopyBatch loads in flight before it stores.  Done one query
   // after another, each copy waited for its own loads (about 0.4 ms of the
-  // launch at 1e4 x 1e6 x 128).
+  // launch at 1e4 x 1e6 x 128).  Only the list's entries are written, and
+  // their count (the merge reads no further).
 #pragma unroll
   for (int g = 0; g < QG; ++g) cnt_s[2 * (wq0 + g * 32 + c) + h] = cnt[g];
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -767,6 +787,7 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
       a.part_d[o] = sd[j];
       a.part_i[o] = si[j];
     }
+    if (lane == 0) a.part_n[(size_t)q * a.n_splits + split] = a.kc;
     wave_sync();
   }
   constexpr int kCopyBatch = 8;
@@ -784,8 +805,9 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
       const int n0 = cnt_s[2 * ql], n1 = cnt_s[2 * ql + 1];
       const bool mine = it < items && q < a.n_q && n0 + n1 <= a.kc &&
                         (!kSelected || a.query_active[q] != 0);
-      dst[u] = mine ? ((int64_t)q * a.n_splits + split) * a.kc + j : -1;
+      if (mine && j == 0) a.part_n[(size_t)q * a.n_splits + split] = n0 + n1;
       const bool has = mine && j < n0 + n1;
+      dst[u] = has ? ((int64_t)q * a.n_splits + split) * a.kc + j : -1;
       const int src = ql * a.cap + (j < n0 ? j : half + (j - n0));
       const uint2 pr = has ? cand[src] : make_uint2(0x7f800000u, 0xffffffffu);
       vd[u] = __uint_as_float(pr.x);
