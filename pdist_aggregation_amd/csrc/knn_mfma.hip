@@ -850,10 +850,12 @@ int occupancy_one(int cap) {
 
 static_assert(kMfmaRowStep == kBY, "row step");
 
-// Query groups (of 32) per wave: two where the resident queries fit in the
-// register budget of 2 waves/SIMD next to two accumulator sets, else one.
+// Query groups (of 32) per wave: two where the resident queries (B operand,
+// 2 x DM x esize bytes per lane pair) leave room for 3 waves per SIMD, else
+// one.  FP8 at DM = 256 takes two (161 VGPRs; 1e4 x 1e6 x 256 search 3.22 ->
+// 2.39 ms against one group); BF16 at 256 and float32 from 128 would not fit.
 constexpr int groups_for(int fmt, int dm) {
-  return ((fmt == kFmtF32 && dm >= 128) || dm >= 256) ? 1 : 2;
+  return ((fmt == kFmtF32 && dm >= 128) || (fmt == kFmtBF16 && dm >= 256)) ? 1 : 2;
 }
 
 int mfma_block_queries(int fmt, int dm) { return 4 * 32 * groups_for(fmt, dm); }
