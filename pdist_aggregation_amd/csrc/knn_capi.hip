@@ -108,7 +108,7 @@ struct Plan {
          off_xerr = 0, off_xscale = 0, off_yscale = 0, off_slots = 0, off_sample = 0,
          off_guess = 0, off_failed = 0, off_failed_qb = 0, off_uncert = 0, off_xpad = 0,
          off_ypad = 0, off_widen = 0, off_widen_qb = 0, off_ubound = 0, off_coll_cnt = 0,
-         off_coll_idx = 0, off_part_n = 0, total = 0;
+         off_coll_idx = 0, off_part_n = 0, off_order = 0, total = 0;
 };
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
@@ -235,6 +235,9 @@ constexpr int kCollMin = 1024, kCollMax = 16384;
 // Collect launch geometry: per-lane list capacity (two halves per query) and
 // the shortest split it is given.
 constexpr int kCollectCap = 64;
+// Collect-list capacity from which the exact re-rank of widened FP8 queries
+// runs in cluster order.
+constexpr int kOrderMinColl = 4096;
 constexpr int64_t kCollectMinRows = 4096;
 
 // runs: sampled runs per (query, region) the kernel writes (2 for the MFMA
@@ -484,6 +487,13 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     p.off_ubound = take((size_t)nq1 * sizeof(double));
     p.off_coll_cnt = take((size_t)nq1 * sizeof(int));
     p.off_coll_idx = take((size_t)nq1 * p.coll_cap * sizeof(int));
+    // FP8 with large collect lists (widening is the rule there, and sets of
+    // thousands of rows overlap between queries of one cluster): the re-rank's
+    // query order, its keys and bucket counts.  On the FP8 mixture
+    // (5e4 x 2e6 x 256, k' = 400) the re-rank takes 13.4 instead of 15.1 ms
+    // for 0.4 ms of ordering; at 1e4 x 1e6 x 128, k' = 40, it gains nothing.
+    if (p.path == kPathMfmaFP8 && p.coll_cap >= kOrderMinColl)
+      p.off_order = take((2 * (size_t)nq1 + knn::kOrderBuckets + 1) * sizeof(int));
   }
   if (p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8) {
     const size_t qe = p.path == kPathMfmaBF16 ? 2 : 1;
@@ -966,6 +976,16 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       cr.uncert = ma.uncert;
       cr.dist_out = dist_out;
       cr.idx_out = idx_out;
+      cr.order = nullptr;
+      if (p.off_order) {
+        // Queries whose collected sets overlap (FP8: whole clusters) re-ranked
+        // side by side on one XCD, which then serves the shared rows from L2.
+        int* key = reinterpret_cast<int*>(ws + p.off_order);
+        int* order = key + n_test;
+        knn::launch_collect_order(ma.widen, ca.coll_cnt, ca.coll_idx, p.coll_cap, n_test,
+                                  n_train, key, order + n_test, order, stream);
+        cr.order = order;
+      }
       knn::launch_collect_rerank<T>(cr, stream);
       if (stats) knn::launch_count_nonzero_i32(ma.widen, n_test, &sc->n_widened, stream);
     }

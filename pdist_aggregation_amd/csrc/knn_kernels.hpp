@@ -299,9 +299,21 @@ struct CollectArgs {
   unsigned char* uncert;
   T* dist_out;
   int64_t* idx_out;
+  // Order of the queries (nullptr: 0 .. n_q-1): workgroup b takes query
+  // order[p], p = b's position in an XCD-major numbering (the XCD takes a
+  // contiguous range of positions), so that queries next to each other in
+  // `order` run together on one XCD and share its L2.
+  const int* order;
 };
 template <class T>
 void launch_collect_rerank(const CollectArgs<T>& a, hipStream_t s);
+// order[0, n_q): the queries bucketed by the smallest row index each widened
+// query collected (queries whose sets overlap mostly share it), widened ones
+// first.  key, counts: scratch of n_q and kOrderBuckets + 1 ints.
+constexpr int kOrderBuckets = 16384;
+void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll_idx,
+                          int coll_cap, int64_t n_q, int64_t n_t, int* key, int* counts,
+                          int* order, hipStream_t s);
 // *out += number of nonzero entries of f[0, n).
 void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s);
 // sc->n_uncertified += number of nonzero flags in uncert[0, n).

@@ -652,7 +652,15 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
 template <class T, int NT>
 __global__ __launch_bounds__(NT) void collect_rerank_kernel(CollectArgs<T> a) {
   extern __shared__ __align__(16) unsigned char dyn_smem[];
-  const int64_t q = blockIdx.x;
+  int64_t q = blockIdx.x;
+  if (a.order) {
+    // Workgroups go to the 8 XCDs round robin: position p = the b-th slot of
+    // XCD b % 8's contiguous range, so neighbours in `order` share an L2.
+    const int64_t per_xcd = ((int64_t)gridDim.x + 7) / 8;
+    const int64_t p = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (p >= a.n_q) return;
+    q = a.order[p];
+  }
   if (a.widen[q] == 0) return;
   const int cnt = a.coll_cnt[q];
   if (cnt > a.coll_cap) return;  // set overflowed: the first answer and its flag stay
@@ -854,8 +862,92 @@ template <class T>
 void launch_collect_rerank(const CollectArgs<T>& a, hipStream_t s) {
   if (a.n_q <= 0) return;
   const size_t lds = (size_t)a.surv_cap * (sizeof(double) + sizeof(int));
-  hipLaunchKernelGGL((collect_rerank_kernel<T, kThreads>), dim3((unsigned)a.n_q), dim3(kThreads),
+  // With an order, the grid is padded to a multiple of 8 (whole XCD ranges).
+  const int64_t blocks = a.order ? (a.n_q + 7) / 8 * 8 : a.n_q;
+  hipLaunchKernelGGL((collect_rerank_kernel<T, kThreads>), dim3((unsigned)blocks), dim3(kThreads),
                      lds, s, a);
+}
+
+namespace {
+
+// key[q] = the smallest row index widened query q collected (n_t for a query
+// that is not widened or whose set overflowed: it sorts last); one wave per
+// query.
+__global__ __launch_bounds__(kThreads) void collect_key_kernel(const int* widen, const int* cnt,
+                                                               const int* idx, int cap,
+                                                               int64_t n_q, int64_t n_t,
+                                                               int* key) {
+  const int lane = threadIdx.x & 63;
+  const int64_t q = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (q >= n_q) return;  // whole wave
+  const int c = widen[q] ? cnt[q] : cap + 1;
+  int m = (int)n_t;
+  if (c <= cap)
+    for (int j = lane; j < c; j += 64) m = min(m, idx[(size_t)q * cap + j]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+  if (lane == 0) key[q] = m < 0 ? 0 : m;
+}
+
+__device__ inline int order_bucket(int key, int64_t n_t) {
+  return (int)((int64_t)key * kOrderBuckets / (n_t + 1));  // key = n_t -> last bucket or below
+}
+
+__global__ void order_count_kernel(const int* key, int64_t n_q, int64_t n_t, int* counts) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_q;
+       q += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&counts[order_bucket(key[q], n_t)], 1);
+}
+
+// Exclusive prefix sum of counts[0, kOrderBuckets) in place, one workgroup of
+// 1024 threads, kOrderBuckets / 1024 consecutive buckets per thread.
+__global__ __launch_bounds__(1024) void order_scan_kernel(int* counts) {
+  constexpr int kPer = kOrderBuckets / 1024;
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  int v[kPer], s = 0;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    v[i] = counts[t * kPer + i];
+    s += v[i];
+  }
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+    const int x = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  int run = part[t] - s;  // exclusive prefix of this thread's run
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    counts[t * kPer + i] = run;
+    run += v[i];
+  }
+}
+
+__global__ void order_scatter_kernel(const int* key, int64_t n_q, int64_t n_t, int* offsets,
+                                     int* order) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_q;
+       q += (int64_t)gridDim.x * blockDim.x)
+    order[atomicAdd(&offsets[order_bucket(key[q], n_t)], 1)] = (int)q;
+}
+
+}  // namespace
+
+void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll_idx,
+                          int coll_cap, int64_t n_q, int64_t n_t, int* key, int* counts,
+                          int* order, hipStream_t s) {
+  if (n_q <= 0) return;
+  hipLaunchKernelGGL(collect_key_kernel, dim3((unsigned)ceil_div(n_q, kThreads / 64)),
+                     dim3(kThreads), 0, s, widen, coll_cnt, coll_idx, coll_cap, n_q, n_t, key);
+  launch_fill_u32(reinterpret_cast<unsigned*>(counts), kOrderBuckets + 1, 0u, s);
+  const unsigned g = (unsigned)std::min<int64_t>(1024, ceil_div(n_q, (int64_t)kThreads));
+  hipLaunchKernelGGL(order_count_kernel, dim3(g), dim3(kThreads), 0, s, key, n_q, n_t, counts);
+  hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, s, counts);
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(g), dim3(kThreads), 0, s, key, n_q, n_t, counts,
+                     order);
 }
 
 void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s) {
