@@ -202,19 +202,27 @@ double binom_tail(int n, double f, int j) {
   return std::max(0.0, 1.0 - cdf);
 }
 
-// Sampled-threshold plan.  The k'-1 rows ranked above a query's k'-th value
+// Sampled-threshold plan.  The guess must leave `need` rows at or below it:
+// k' where the merge needs k' candidates (generic and low-d kernels), k for
+// the MFMA kernels, whose merge certifies a query with k <= n < k'
+// candidates against the guess itself (every row above it was skipped, none
+// at or below it).  The need-1 rows ranked above a query's need-th value
 // each fall in a sample of fraction f with probability f (rows in random
-// order), so the j-th smallest sampled value lies below the k'-th value --
-// the guess is too tight -- with probability P(Bin(k'-1, f) >= j).  Two
+// order), so the j-th smallest sampled value lies below the need-th value --
+// the guess is too tight -- with probability P(Bin(need-1, f) >= j).  Two
 // reductions only raise the guess, never lower it: the sample keeps minima
 // of 16 rows, and each sampling lane keeps only its kSampleJ smallest
 // minima (so j may exceed kSampleJ: the j-th smallest of the kept values is
 // at least the j-th smallest of all).  j is the smallest with that bound at
 // most kGuessFail, up to kGuessMaxJ rounds of the guess kernel; a query whose
-// guess fails is searched again, so the bound only governs speed.
+// guess fails is searched again, so the bound only governs speed.  A guess
+// about j / f rows deep lets that many rows through the search: at
+// 1e4 x 1e6 x 128 (k = 10, k' = 40, f = 1/32) j = 6 against k instead of
+// j = 11 against k' cuts them from about 350 to about 190 per query.
 // PDIST_KNN_GUESS: "0" off, "1" on at any size, "fail:S" on with the guess of
 // every S-th query forced to fail (exercises the re-search; "fail" = every
-// query).
+// query), "j:J" on with j = J (a tight guess exercises the certification of
+// queries with fewer than k' candidates).
 constexpr int64_t kGuessMinRows = 32768;
 // Generic kernel: workgroups its search grid aims for, and the list capacity
 // of its sampling launch (kSampleJ kept + one 64-row step of appends).
@@ -242,13 +250,14 @@ constexpr int64_t kCollectMinRows = 4096;
 
 // runs: sampled runs per (query, region) the kernel writes (2 for the MFMA
 // kernel's two lane halves, 1 for the low-d kernel's one lane per query).
-void plan_guess(Plan& p, int64_t slots, int runs) {
+void plan_guess(Plan& p, int64_t slots, int runs, int64_t need) {
   const char* env = test_switch("PDIST_KNN_GUESS");
   const std::string mode = env ? env : "";
-  int fail_stride = 0;
+  int fail_stride = 0, forced_j = 0;
   if (mode == "fail") fail_stride = 1;
   if (mode.rfind("fail:", 0) == 0) fail_stride = std::max(1, std::atoi(mode.c_str() + 5));
-  const bool forced = mode == "1" || fail_stride > 0;
+  if (mode.rfind("j:", 0) == 0) forced_j = std::max(1, std::atoi(mode.c_str() + 2));
+  const bool forced = mode == "1" || fail_stride > 0 || forced_j > 0;
   // Off below kGuessMinRows train rows (the sampling launch and the guess
   // cost more than they save), unless a split holds fewer than 64 k' rows:
   // started from +inf, its lists would fill and be compacted over and over.
@@ -273,15 +282,16 @@ void plan_guess(Plan& p, int64_t slots, int runs) {
     p.sample_stride = stride * knn::kMfmaRowStep;
     p.sample_fraction = std::min(1.0, (double)(regions * per) / (double)tiles);
     j = 1;
-    while (j <= kGuessMaxJ && binom_tail(p.kc - 1, p.sample_fraction, j) > kGuessFail) ++j;
+    while (j <= kGuessMaxJ && binom_tail(need - 1, p.sample_fraction, j) > kGuessFail) ++j;
   }
   const int jmax = (int)std::min<int64_t>(kGuessMaxJ, runs * regions * knn::kSampleJ);
+  if (forced_j > 0) j = forced_j;
   if (j > jmax) {
     if (!forced) return;
     j = jmax;
   }
   p.sample_j = j;
-  p.guess_fail_bound = binom_tail(p.kc - 1, p.sample_fraction, j);
+  p.guess_fail_bound = binom_tail(need - 1, p.sample_fraction, j);
   p.guess = true;
   p.guess_fail_stride = fail_stride;
 }
@@ -429,14 +439,14 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     p.n_groups = (int)knn::ceil_div((int64_t)p.n_splits, (int64_t)p.group_size);
   }
   const bool mfma = p.path == kPathMfmaF32 || p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8;
-  if (mfma) plan_guess(p, slots, 2);
-  if (p.path == kPathLowd) plan_guess(p, slots, 1);
+  if (mfma) plan_guess(p, slots, 2, p.k);
+  if (p.path == kPathLowd) plan_guess(p, slots, 1, p.kc);
   // Generic kernel: one run per (query, region), from as many sampling
   // workgroups as the search's own grid target.  Only with 4 or more splits:
   // the guess saves each split's first fill and compaction, and with fewer
   // the sampling launch costs more than those (float64 1e5 queries x 1e3
   // rows x 100, k = 10, one split: 2.6 -> 2.9 ms with it).
-  if (p.path == kPathGeneric && p.n_splits >= 4) plan_guess(p, kGenericGridTarget, 1);
+  if (p.path == kPathGeneric && p.n_splits >= 4) plan_guess(p, kGenericGridTarget, 1, p.kc);
 
   const size_t celem = (p.path == kPathGeneric || p.path == kPathLowd) ? elem : sizeof(float);
   const int64_t nq1 = std::max<int64_t>(n_test, 1);
@@ -647,6 +657,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.n_groups = 1;
   ma.failed = nullptr;
   ma.failed_qblocks = nullptr;
+  ma.guess_bound = 0;
   ma.qblock_queries = 1;
   ma.tau = tau;
   ma.only_queries = nullptr;
@@ -849,6 +860,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       }
       ma.failed = failed;
       ma.failed_qblocks = failed_qb;
+      ma.guess_bound = 1;
       ma.qblock_queries = knn::mfma_block_queries(fa.fmt, p.dm);
       knn::launch_sample_guess(sa.sample_out, n_test, 2 * p.sample_regions, p.sample_j, tau,
                                reinterpret_cast<float*>(ws + p.off_guess), p.guess_fail_stride,

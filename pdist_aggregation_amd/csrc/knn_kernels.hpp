@@ -249,6 +249,14 @@ struct MergeArgs {
   // outputs are left to the re-search.
   int* failed;
   int* failed_qblocks;
+  // With failed set: 1 if the guess was planned for k rather than k' rows
+  // (MFMA kernels).  A query then fails only below min(k, n_t) candidates;
+  // one with k <= n < k' candidates lost no row at or below its threshold
+  // tau[q] (the guess: with fewer than k' rows under it no list reached k'
+  // entries, so no compaction or shared bound lowered it) and every row it
+  // lost lies above it, so the certification takes that threshold in place
+  // of the k'-th value.
+  int guess_bound;
   int qblock_queries;
   unsigned long long* tau;
   // Re-search merge: only queries with only_queries[q] != 0 (nullptr: all).

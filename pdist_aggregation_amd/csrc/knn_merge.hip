@@ -458,9 +458,12 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
 
   // Sampled-threshold check: with at least min(k', n_t) candidates every row
   // the guess dropped has a value above the k'-th kept one, so nothing is
-  // lost; with fewer, the query is searched again from an infinite threshold.
+  // lost; with fewer (fewer than min(k, n_t) when the guess was planned for
+  // k: see guess_bound), the query is searched again from an infinite
+  // threshold.
   if (a.failed) {
-    const int need = (int64_t)a.kc < a.n_t ? a.kc : (int)a.n_t;
+    const int want = a.guess_bound ? a.k : a.kc;
+    const int need = (int64_t)want < a.n_t ? want : (int)a.n_t;
     if (n < need) {  // n_s: the same value for every thread
       if (tid == 0) {
         a.failed[q] = 1;
@@ -474,6 +477,12 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
   if (tid == 0) {
     full_s = oi[a.kc - 1] != INT_MAX;
     akc_s = full_s ? (double)od[a.kc - 1] : 0.0;
+    // Fewer than k' candidates under a guess planned for k (n >= k here):
+    // every row left out has a value above the threshold.
+    if (!full_s && a.failed && a.guess_bound && tq < pos_inf<T>()) {
+      full_s = 1;
+      akc_s = (double)tq;
+    }
   }
   __syncthreads();  // od is overwritten below (it may share storage with ed)
 

@@ -3,10 +3,12 @@
 
 The search first sweeps 1/32 of the train rows, keeps per-lane minima
 and starts every query from a guessed threshold (the j-th smallest sampled
-minimum).  A query left with fewer than k' candidates is searched again from
-an infinite threshold.  PDIST_KNN_GUESS forces the sampling on at small sizes
-("1") and forces chosen guesses to fail ("fail", "fail:S" = every S-th
-query), so both the normal and the re-search path are compared with the FP64
+minimum).  A query left with fewer than k' candidates (MFMA kernels: fewer
+than k; with k to k'-1 the merge certifies it against the guess) is searched
+again from an infinite threshold.  PDIST_KNN_GUESS forces the sampling on at
+small sizes ("1"), forces chosen guesses to fail ("fail", "fail:S" = every
+S-th query) and forces the guess's rank ("j:J"), so the normal path, the
+short candidate sets and the re-search path are compared with the FP64
 oracle here.
 """
 import sys
@@ -81,6 +83,27 @@ def test_every_guess_failing_is_searched_again(require_gpu, monkeypatch, precisi
     dist, idx, st = _run(Y, X, k, precision)
     assert st["n_rescued"] == X.shape[0]
     _check(X, Y, dist, idx, k, precision, st)
+
+
+@pytest.mark.parametrize("precision,d", [("exact", 64), ("exact", 128), ("bf16", 128),
+                                         ("bf16", 64), ("fp8", 128)])
+@pytest.mark.parametrize("j", [1, 2])
+def test_tight_guess_short_candidate_sets(require_gpu, monkeypatch, precision, d, j):
+    """MFMA kernels with the guess forced to the 1st / 2nd smallest sampled
+    minimum, about 32 / 64 rows deep: most queries end with k to k'-1
+    candidates (k = 10, k' = 40 for BF16 / FP8, 18 for float32) and are
+    certified against the guess or widened, the rest (fewer than k) are
+    searched again; every answer is the oracle's."""
+    monkeypatch.setenv("PDIST_KNN_GUESS", f"j:{j}")
+    rs = np.random.RandomState(d * 3 + j)
+    Y = rs.rand(40000, d).astype(np.float32)
+    X = rs.rand(700, d).astype(np.float32)
+    _say(f"[guess j:{j}] {precision} d={d} k=10 40000 x 700")
+    dist, idx, st = _run(Y, X, 10, precision)
+    _say(f"    stats {st}")
+    assert st["kernel_path"] in (1, 2, 3)
+    assert 0 < st["n_rescued"] < X.shape[0]
+    _check(X, Y, dist, idx, 10, precision, st)
 
 
 @pytest.mark.parametrize("stride", [500, 3, 64])
