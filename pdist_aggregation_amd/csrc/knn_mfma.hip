@@ -609,7 +609,12 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   };
   // Threshold refresh (search launches; the collect launch keeps its fixed
   // threshold), called by the sweep after ring cycles 1, 2, 5 and then every
-  // 11 (every 33 rows steps: cheap early sharing, rare later).
+  // 11 (every 33 rows steps: cheap early sharing, rare later) when splits
+  // publish slots, else after cycles 1, 3, 7, 15, ...: with a sampled guess
+  // only a compaction that reaches k' entries lowers the shared threshold,
+  // and each refresh waits for the wave's tile loads in flight (the load's
+  // wait cannot leave them out), which at 1e4 x 1e6 x 128 cost 1.6 % of the
+  // search at the denser schedule.
   auto refresh = [&]() {
     if constexpr (MODE == kModeSearch || MODE == kModeResearch) {
       // Thresholds shared across the splits of a query:
@@ -758,7 +763,7 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   sweep(select, [&](int cyc) {
     if (cyc == next_refresh) {
       refresh();
-      next_refresh = cyc == 1 ? 2 : cyc == 2 ? 5 : cyc + 11;
+      next_refresh = !a.qslots ? 2 * cyc + 1 : cyc == 1 ? 2 : cyc == 2 ? 5 : cyc + 11;
     }
   });
   if constexpr (MODE == kModeCollect) {
