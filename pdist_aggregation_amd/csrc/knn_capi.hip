@@ -108,7 +108,7 @@ struct Plan {
          off_xerr = 0, off_xscale = 0, off_yscale = 0, off_slots = 0, off_sample = 0,
          off_guess = 0, off_failed = 0, off_failed_qb = 0, off_uncert = 0, off_xpad = 0,
          off_ypad = 0, off_widen = 0, off_widen_qb = 0, off_ubound = 0, off_coll_cnt = 0,
-         off_coll_idx = 0, off_part_n = 0, off_order = 0, total = 0;
+         off_coll_idx = 0, off_part_n = 0, off_order = 0, off_qmap = 0, total = 0;
 };
 
 constexpr int kPathGeneric = 0, kPathMfmaF32 = 1, kPathMfmaBF16 = 2, kPathMfmaFP8 = 3,
@@ -497,6 +497,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     p.off_ubound = take((size_t)nq1 * sizeof(double));
     p.off_coll_cnt = take((size_t)nq1 * sizeof(int));
     p.off_coll_idx = take((size_t)nq1 * p.coll_cap * sizeof(int));
+    p.off_qmap = take(((size_t)nq1 + 1) * sizeof(int));  // widened queries, then their count
     // FP8 with large collect lists (widening is the rule there, and sets of
     // thousands of rows overlap between queries of one cluster): the re-rank's
     // query order, its keys and bucket counts.  On the FP8 mixture
@@ -749,6 +750,8 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     fa.split_stride = p.rows_per_split;
     fa.qblock_active = nullptr;
     fa.query_active = nullptr;
+    fa.qmap = nullptr;
+    fa.qmap_n = nullptr;
     fa.sample_out = nullptr;
     fa.coll_cnt = nullptr;
     fa.coll_idx = nullptr;
@@ -760,7 +763,10 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     ma.qblock_queries = knn::mfma_block_queries(
         p.path == kPathMfmaF32 ? knn::kFmtF32 : p.path == kPathMfmaBF16 ? knn::kFmtBF16 : knn::kFmtFP8,
         p.dm);
-    if (const char* w = test_switch("PDIST_KNN_WIDEN")) ma.widen_all = std::strcmp(w, "all") == 0;
+    if (const char* w = test_switch("PDIST_KNN_WIDEN")) {
+      if (std::strcmp(w, "all") == 0) ma.widen_all = 1;
+      if (std::strncmp(w, "every:", 6) == 0) ma.widen_all = std::max(1, std::atoi(w + 6));
+    }
     knn::launch_fill_u32(reinterpret_cast<unsigned*>(ma.widen), n_test, 0u, stream);
     knn::launch_fill_u32(reinterpret_cast<unsigned*>(ma.widen_qblocks), p.n_qblocks, 0u, stream);
     fa.cand_d = reinterpret_cast<float*>(ws + p.off_cand_d);
@@ -970,6 +976,16 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
         ca.n_splits = (int)knn::ceil_div(n_train, ca.rows_per_split);
       }
       knn::launch_fill_u32(reinterpret_cast<unsigned*>(ca.coll_cnt), n_test, 0u, stream);
+      // The widened queries packed into the first query blocks: a few of
+      // them take one block's sweep instead of one per block they fall in
+      // (BF16, 100 Gaussians, 1e5 x 1e6 x 128: 54 widened queries in about
+      // 50 blocks).
+      {
+        int* qmap = reinterpret_cast<int*>(ws + p.off_qmap);
+        knn::launch_pack_flagged(ma.widen, n_test, qmap, qmap + n_test, stream);
+        ca.qmap = qmap;
+        ca.qmap_n = qmap + n_test;
+      }
       knn::launch_mfma_collect(ca, stream);
       knn::CollectArgs<T> cr;
       cr.X = X_test;

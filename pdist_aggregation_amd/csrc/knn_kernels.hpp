@@ -122,6 +122,15 @@ struct MfmaArgs {
   int* coll_cnt;
   int* coll_idx;
   int coll_cap;
+  // Packed query blocks, read by the collect launch only (nullptr: query
+  // block qb holds queries qb * BQ + i).  While 2 * *qmap_n < n_q, query
+  // block qb holds queries qmap[qb * BQ + i] for qb * BQ + i < *qmap_n and
+  // blocks past *qmap_n exit at once; with more, the launch runs unpacked
+  // (qblock_active and query_active must be set either way).  The collect
+  // launch packs the widened queries this way, so a few of them cost one
+  // block's sweep, not one per query block they fall in.
+  const int* qmap;
+  const int* qmap_n;
 };
 
 constexpr int kSlotJ = 4;
@@ -275,7 +284,8 @@ struct MergeArgs {
   // ubound[q] = its k-th exact squared distance U, and tau[q] = the key of T,
   // the largest value the search can compute for a row within distance
   // sqrt(U) of the query: a collect launch then gathers every such row.
-  // widen_all (testing) widens every query that is not trivially certified.
+  // widen_all (testing): S > 0 widens every S-th query (q % S == 0) that is
+  // not trivially certified.
   int* widen;
   int* widen_qblocks;
   double* ubound;
@@ -324,6 +334,9 @@ void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll
                           int* order, hipStream_t s);
 // *out += number of nonzero entries of f[0, n).
 void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s);
+// list[0 .. *count) = the q < n with flags[q] != 0, in no particular order
+// (*count zeroed here first).
+void launch_pack_flagged(const int* flags, int64_t n, int* list, int* count, hipStream_t s);
 // sc->n_uncertified += number of nonzero flags in uncert[0, n).
 void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
                               hipStream_t s);

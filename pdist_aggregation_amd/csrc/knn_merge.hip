@@ -628,7 +628,7 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
       const double lower = sqrt(lo > 0.0 ? lo : 0.0) - ex - ey;
       certified = lower > sqrt(kth_s);
     }
-    if (a.widen_all && !trivial) certified = false;
+    if (a.widen_all > 0 && q % a.widen_all == 0 && !trivial) certified = false;
     a.uncert[q] = certified ? 0 : 1;
     // Widening: a train row y within the k-th exact distance sqrt(U) of x
     // (every true neighbour is: U bounds the true k-th distance from above)
@@ -957,6 +957,36 @@ void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll
   hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, s, counts);
   hipLaunchKernelGGL(order_scatter_kernel, dim3(g), dim3(kThreads), 0, s, key, n_q, n_t, counts,
                      order);
+}
+
+namespace {
+
+// One atomic per wave reserves the slots of its flagged entries.
+__global__ __launch_bounds__(kThreads) void pack_flagged_kernel(const int* flags, int64_t n,
+                                                                int* list, int* count) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = (int64_t)blockIdx.x * kThreads; base < n;
+       base += (int64_t)gridDim.x * kThreads) {
+    const int64_t q = base + threadIdx.x;
+    const bool on = q < n && flags[q] != 0;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(on);
+    if (m == 0) continue;  // uniform across the wave
+    int at = 0;
+    if (lane == 0) at = atomicAdd(count, __popcll(m));
+    at = __shfl(at, 0, 64);
+    const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+    if (on) list[at + rank] = (int)q;
+  }
+}
+
+}  // namespace
+
+void launch_pack_flagged(const int* flags, int64_t n, int* list, int* count, hipStream_t s) {
+  launch_fill_u32(reinterpret_cast<unsigned*>(count), 1, 0u, s);
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>(1024, ceil_div(n, (int64_t)kThreads));
+  hipLaunchKernelGGL(pack_flagged_kernel, dim3(g), dim3(kThreads), 0, s, flags, n, list, count);
 }
 
 void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s) {

@@ -9,7 +9,9 @@ every query, so the path is compared with the FP64 oracle here on small
 inputs for every MFMA format, with exact ties at the k-th distance, together
 with failing sampled guesses (the re-search's merge widens too), and with a
 collect set larger than its capacity (the query keeps its first answer and
-stays counted as uncertified).
+stays counted as uncertified).  PDIST_KNN_WIDEN=every:S widens every S-th
+query: fewer than half of them run the collect launch in packed query
+blocks, more run it in the search's own blocks.
 """
 import sys
 
@@ -50,6 +52,26 @@ def test_widen_every_query_matches_oracle(require_gpu, monkeypatch, precision, d
     assert st["n_uncertified"] == 0
     assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
     assert_knn_parity(X, Y, dist, idx, k, EPS_F32)
+
+
+@pytest.mark.parametrize("precision,d", [("exact", 64), ("bf16", 128), ("fp8", 128),
+                                         ("fp8", 256)])
+@pytest.mark.parametrize("stride", [1000, 7, 2])
+def test_widen_some_queries_packed_or_not(require_gpu, monkeypatch, precision, d, stride):
+    """Every stride-th query widened: one (stride 1000) or 1/7 of the queries
+    (packed blocks), or half of them (the search's blocks); 1,100 queries
+    span several query blocks."""
+    monkeypatch.setenv("PDIST_KNN_WIDEN", f"every:{stride}")
+    rs = np.random.RandomState(d + stride)
+    Y = rs.rand(30000, d).astype(np.float32)
+    X = rs.rand(1100, d).astype(np.float32)
+    _say(f"[widen every:{stride}] {precision} d={d} k=10 30000 x 1100")
+    dist, idx, st = _run(Y, X, 10, precision)
+    _say(f"    stats {st}")
+    assert st["n_widened"] >= len(range(0, X.shape[0], stride))
+    assert st["n_uncertified"] == 0
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, 10, EPS_F32)
 
 
 @pytest.mark.parametrize("precision", ["exact", "bf16", "fp8"])
