@@ -234,11 +234,13 @@ static_assert(kGenericSampleCap >= knn::kSampleJ + knn::kGenY &&
 constexpr int kSampleShift = 5;  // sample 1/32 of the train rows
 constexpr double kGuessFail = 1e-7;
 constexpr int kGuessMaxJ = 128;
-// Collect lists: room for 8 k' rows per query, at least 1024 and at most
-// 16384.  A CPU model of the quantisation puts the bound-following set at
-// about k + 2 rows for BF16 on the 1e4 x 1e6 x 128 U[0,1) benchmark inputs,
-// 100-350 rows for FP8 there, and about 2,000 (a whole Gaussian) for FP8 on
-// the 1,000-Gaussian mixture in d = 256 with k' = 400 (capacity 4096).
+// Collect lists: room for 8 k' rows per query (FP8: 16 k'), at least 1024
+// and at most 16384.  A CPU model of the quantisation puts the
+// bound-following set at about k + 2 rows for BF16 on the 1e4 x 1e6 x 128
+// U[0,1) benchmark inputs, 100-350 rows for FP8 there, and about 2,000 (a
+// whole Gaussian) for FP8 on the 1,000-Gaussian mixture in d = 256 with
+// k' = 400 (capacity 8192).  FP8 with k = 32 (k' = 128) on 1e5 U[0,1)
+// queries overflowed 1,024 rows for 7 queries (left uncertified).
 constexpr int kCollMin = 1024, kCollMax = 16384;
 // Collect launch geometry: per-lane list capacity (two halves per query) and
 // the shortest split it is given.
@@ -491,7 +493,8 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     p.off_ypad = take((size_t)n_train * p.dpad * sizeof(float));
   }
   if (mfma) {
-    p.coll_cap = std::min(kCollMax, knn::next_pow2(std::max(kCollMin, 8 * p.kc)));
+    p.coll_cap = std::min(kCollMax, knn::next_pow2(std::max(
+                                        kCollMin, (p.path == kPathMfmaFP8 ? 16 : 8) * p.kc)));
     p.off_widen = take((size_t)nq1 * sizeof(int));
     p.off_widen_qb = take((size_t)p.n_qblocks * sizeof(int));
     p.off_ubound = take((size_t)nq1 * sizeof(double));
