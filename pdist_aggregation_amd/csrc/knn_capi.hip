@@ -90,6 +90,7 @@ struct Plan {
   int64_t bigk_chunk = 0;
   size_t off_bigk_d = 0, off_bigk_sel_d = 0, off_bigk_sel_i = 0;
   int64_t rows_per_split = 0, n_qblocks = 0;
+  int64_t slots = 0;  // resident search workgroups the device holds (MFMA, low-d)
   size_t elem = 4;
   // Sampled threshold (MFMA paths): a first launch over sample_regions row
   // ranges of sample_rows rows, sample_stride apart, estimates each query's
@@ -420,6 +421,7 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
                                                                  : knn::kFmtFP8,
                                        p.dm, p.cap);
     slots = (int64_t)device_cus() * per_cu;
+    p.slots = slots;
     splits = choose_splits_slots(p.n_qblocks, n_train, step, p.kc, slots);
     // The MFMA kernel addresses one split with 32-bit buffer offsets.
     if (p.path != kPathLowd) {
@@ -755,6 +757,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     fa.query_active = nullptr;
     fa.qmap = nullptr;
     fa.qmap_n = nullptr;
+    fa.coll_group = 1;
     fa.sample_out = nullptr;
     fa.coll_cnt = nullptr;
     fa.coll_idx = nullptr;
@@ -977,6 +980,13 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
                             knn::kMfmaRowStep;
         ca.split_stride = ca.rows_per_split;
         ca.n_splits = (int)knn::ceil_div(n_train, ca.rows_per_split);
+        // Unpacked (most queries widened), the search's own split count when
+        // its workgroups fit one round (FP8 at 1e4 x 1e6 x 128: collect
+        // 1.28 -> 1.09 ms); with several rounds the short splits balance the
+        // rounds better (FP8 on 1,000 Gaussians, 3.8 rounds: 11 ms slower
+        // grouped).
+        if (p.n_qblocks * p.n_splits <= p.slots)
+          ca.coll_group = (int)std::max<int64_t>(1, ca.n_splits / std::max(1, p.n_splits));
       }
       knn::launch_fill_u32(reinterpret_cast<unsigned*>(ca.coll_cnt), n_test, 0u, stream);
       // The widened queries packed into the first query blocks: a few of

@@ -131,6 +131,9 @@ struct MfmaArgs {
   // block's sweep, not one per query block they fall in.
   const int* qmap;
   const int* qmap_n;
+  // Collect launch, unpacked: workgroups take coll_group consecutive splits
+  // each (1: one split); split_stride must equal rows_per_split then.
+  int coll_group;
 };
 
 constexpr int kSlotJ = 4;

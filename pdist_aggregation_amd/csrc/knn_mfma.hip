@@ -194,17 +194,6 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   // round-robin; work items are numbered split-major and each XCD takes a
   // contiguous range of them, so the query blocks of one split run together
   // on one XCD and share its L2 while they stream the same train rows.
-  int qb, split;
-  {
-    const int b = blockIdx.x;
-    const int per_xcd = (int)(gridDim.x >> 3);
-    const int w_item = (b & 7) * per_xcd + (b >> 3);
-    if (w_item >= a.n_qblocks * a.n_splits) return;  // grid padding (whole workgroup)
-    split = w_item / a.n_qblocks;
-    qb = w_item % a.n_qblocks;
-  }
-  constexpr bool kSelected = MODE == kModeResearch || MODE == kModeCollect;
-  const int64_t q0 = (int64_t)qb * BQ;
   // Packed blocks (collect launch only): slot q0 + i holds query
   // qmap[q0 + i], while fewer than half of all queries are listed (the count
   // is the same for every workgroup, so all take the same layout); with more,
@@ -214,14 +203,37 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   const int* qmap = MODE == kModeCollect ? a.qmap : nullptr;
   const int64_t n_mapped = qmap ? (int64_t)__builtin_amdgcn_readfirstlane(*a.qmap_n) : 0;
   if (qmap && 2 * n_mapped >= a.n_q) qmap = nullptr;
+  // Unpacked collect launch: the grid is sized for short splits (the latency
+  // of a few packed blocks), so consecutive groups of coll_group splits go
+  // to one workgroup each, which keeps the query loads per row what the
+  // search's are (the leading workgroups, dealt to the XCDs as below).
+  const int group = MODE == kModeCollect && !qmap && a.coll_group > 1 ? a.coll_group : 1;
+  int qb, split, nsplit = 1;
+  {
+    const int b = blockIdx.x;
+    const int groups = (a.n_splits + group - 1) / group;
+    const int items = a.n_qblocks * groups;
+    const int grid = group == 1 ? (int)gridDim.x : (items + 7) & ~7;
+    if (b >= grid) return;  // past the grouped grid (whole workgroup)
+    const int per_xcd = grid >> 3;
+    const int w_item = (b & 7) * per_xcd + (b >> 3);
+    if (w_item >= items) return;  // grid padding (whole workgroup)
+    split = (w_item / a.n_qblocks) * group;
+    qb = w_item % a.n_qblocks;
+    nsplit = a.n_splits - split < group ? a.n_splits - split : group;
+  }
+  constexpr bool kSelected = MODE == kModeResearch || MODE == kModeCollect;
+  const int64_t q0 = (int64_t)qb * BQ;
   if (qmap ? q0 >= n_mapped : kSelected && a.qblock_active[qb] == 0) return;  // not selected
   // Query of local slot ql (0 <= ql < BQ), or -1 for an empty packed slot.
   auto query_of = [&](int ql) -> int64_t {
     if (!qmap) return q0 + ql;
     return q0 + ql < n_mapped ? (int64_t)qmap[q0 + ql] : -1;
   };
+  // A group of splits is contiguous (split_stride == rows_per_split there).
   const int64_t ybeg = (int64_t)split * a.split_stride;
-  const int64_t yend = ybeg + a.rows_per_split < a.n_t ? ybeg + a.rows_per_split : a.n_t;
+  const int64_t ylen = (int64_t)(nsplit - 1) * a.split_stride + a.rows_per_split;
+  const int64_t yend = ybeg + ylen < a.n_t ? ybeg + ylen : a.n_t;
   const size_t blk = (size_t)split * a.n_qblocks + qb;
   // Candidate lists of the search launches: (value bits, row) pairs, one
   // 8-byte store per survivor (two 4-byte stores cost about 0.13 ms more at
