@@ -15,7 +15,9 @@ namespace knn {
 
 namespace {
 
-constexpr int kSampleRowsPerBlock = 256;
+// Rows per column-sum workgroup: 64 keeps each thread's chain of dependent
+// adds short (256 rows per workgroup: 17 us for 65,536 x 128 floats).
+constexpr int kSampleRowsPerBlock = 64;
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // Column means without floating-point atomics, so the centring vector --
