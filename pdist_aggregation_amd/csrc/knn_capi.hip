@@ -649,11 +649,11 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     return finish();
   }
 
-  // 1. fill: scalars are 4 ordered-key slots (max keys start at key(-inf)).
-  knn::launch_fill_keys<float>(reinterpret_cast<unsigned long long*>(sc),
-                               sizeof(knn::DeviceScalars) / 8,
-                               0ull, stream);
-  knn::launch_fill_keys<float>(tau, n_test, knn::ordered_key(__builtin_huge_val()), stream);
+  // 1. fills, gathered into one launch before the path's first kernel:
+  // scalars are 4 ordered-key slots (max keys start at key(-inf)).
+  knn::FillBatch fills;
+  fills.add64(sc, sizeof(knn::DeviceScalars) / 8, 0ull);
+  fills.add64(tau, n_test, knn::ordered_key(__builtin_huge_val()));
   knn::MergeArgs<T> ma;
   ma.unsorted = 0;
   ma.part_n = nullptr;
@@ -670,6 +670,8 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.widen = nullptr;
   ma.widen_qblocks = nullptr;
   ma.ubound = nullptr;
+  ma.qmap = nullptr;
+  ma.qmap_n = nullptr;
   ma.widen_all = 0;
   knn::MfmaArgs fa;
   ma.xnorm = nullptr;
@@ -679,6 +681,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   knn::SearchArgs<T> sa{};  // generic / low-d search (value-initialised: a plain search)
   if (p.path == kPathGeneric || p.path == kPathLowd) {
     // 2. norms  3. search
+    knn::launch_fill_batch(fills, stream);
     knn::launch_row_norms<T>(X_train, n_train, (int)dim, ynorm, sc, stream);
     sa.X = X_test;
     sa.Y = X_train;
@@ -773,8 +776,16 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       if (std::strcmp(w, "all") == 0) ma.widen_all = 1;
       if (std::strncmp(w, "every:", 6) == 0) ma.widen_all = std::max(1, std::atoi(w + 6));
     }
-    knn::launch_fill_u32(reinterpret_cast<unsigned*>(ma.widen), n_test, 0u, stream);
-    knn::launch_fill_u32(reinterpret_cast<unsigned*>(ma.widen_qblocks), p.n_qblocks, 0u, stream);
+    // The widened queries' packed list for the collect launch (filled by the
+    // merges), its count after the n_test entries.
+    ma.qmap = reinterpret_cast<int*>(ws + p.off_qmap);
+    ma.qmap_n = ma.qmap + n_test;
+    fills.add32(ma.widen, n_test, 0u);
+    fills.add32(ma.widen_qblocks, p.n_qblocks, 0u);
+    fills.add32(ma.qmap_n, 1, 0u);
+    fills.add32(ws + p.off_coll_cnt, n_test, 0u);
+    // |y|^2 = +inf past the last train row (the kernels' partial last tile)
+    fills.add32(reinterpret_cast<float*>(ynorm) + n_train, knn::kNormPad, 0x7f800000u);
     fa.cand_d = reinterpret_cast<float*>(ws + p.off_cand_d);
     fa.cand_i = reinterpret_cast<int*>(ws + p.off_cand_i);
     fa.tau = tau;
@@ -791,9 +802,9 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     fa.slot_stride = knn::align8(p.n_splits);
     if (p.n_splits >= knn::kSlotJ && !p.guess) {
       fa.qslots = reinterpret_cast<float*>(ws + p.off_slots);
-      knn::launch_fill_u32(reinterpret_cast<unsigned*>(fa.qslots), n_test * fa.slot_stride,
-                           0x7f800000u, stream);  // +inf
+      fills.add32(fa.qslots, n_test * fa.slot_stride, 0x7f800000u);  // +inf
     }
+    knn::launch_fill_batch(fills, stream);
     if (p.path == kPathMfmaF32) {
       // 2. norms
       knn::launch_row_norms<float>(reinterpret_cast<const float*>(X_train), n_train, (int)dim,
@@ -853,9 +864,6 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       ma.unit_roundoff = 0x1p-24;
       ma.acc_terms = 2 * p.dm + 4;
     }
-    // |y|^2 = +inf past the last train row (the kernels' partial last tile)
-    knn::launch_fill_u32(reinterpret_cast<unsigned*>(ynorm) + n_train, knn::kNormPad,
-                         0x7f800000u, stream);
     // 2b. sampled threshold: minima over 1/32 of the rows -> guess -> tau
     if (p.guess) {
       auto* failed = reinterpret_cast<int*>(ws + p.off_failed);
@@ -988,17 +996,12 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
         if (p.n_qblocks * p.n_splits <= p.slots)
           ca.coll_group = (int)std::max<int64_t>(1, ca.n_splits / std::max(1, p.n_splits));
       }
-      knn::launch_fill_u32(reinterpret_cast<unsigned*>(ca.coll_cnt), n_test, 0u, stream);
-      // The widened queries packed into the first query blocks: a few of
-      // them take one block's sweep instead of one per block they fall in
-      // (BF16, 100 Gaussians, 1e5 x 1e6 x 128: 54 widened queries in about
-      // 50 blocks).
-      {
-        int* qmap = reinterpret_cast<int*>(ws + p.off_qmap);
-        knn::launch_pack_flagged(ma.widen, n_test, qmap, qmap + n_test, stream);
-        ca.qmap = qmap;
-        ca.qmap_n = qmap + n_test;
-      }
+      // The widened queries packed into the first query blocks (listed by
+      // the merges): a few of them take one block's sweep instead of one per
+      // block they fall in (BF16, 100 Gaussians, 1e5 x 1e6 x 128: 54 widened
+      // queries in about 50 blocks).
+      ca.qmap = ma.qmap;
+      ca.qmap_n = ma.qmap_n;
       knn::launch_mfma_collect(ca, stream);
       knn::CollectArgs<T> cr;
       cr.X = X_test;

@@ -2,6 +2,7 @@
 // kernels.  Every pointer is a device pointer; every launch is asynchronous on
 // `stream`.
 #pragma once
+#include <cstdlib>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -158,6 +159,31 @@ void launch_sample_guess(const float* sample_out, int64_t n_q, int n_runs, int j
                          unsigned long long* tau, float* guess, int fail_stride, int* failed,
                          int* failed_qblocks, int qblock_queries, hipStream_t s);
 void launch_fill_u32(unsigned int* p, int64_t n, unsigned int v, hipStream_t s);
+
+// Several fills in one launch (each kernel boundary on a stream costs a few
+// microseconds of idle GPU): segment i sets n[i] elements of 4 or 8 bytes
+// (wide[i]) at p[i] to v[i].
+constexpr int kMaxFills = 10;
+struct FillBatch {
+  void* p[kMaxFills];
+  int64_t n[kMaxFills];
+  unsigned long long v[kMaxFills];
+  int wide[kMaxFills];
+  int count = 0;
+  void add32(void* at, int64_t len, unsigned val) { add(at, len, val, 0); }
+  void add64(void* at, int64_t len, unsigned long long val) { add(at, len, val, 1); }
+  void add(void* at, int64_t len, unsigned long long val, int w) {
+    if (len <= 0) return;
+    if (count == kMaxFills) std::abort();  // the call sites set at most 8
+    p[count] = at;
+    n[count] = len;
+    v[count] = val;
+    wide[count] = w;
+    ++count;
+  }
+};
+// Launches the batch's fills (none: no launch) and empties it.
+void launch_fill_batch(FillBatch& b, hipStream_t s);
 // Queries per workgroup of the (fmt, dm) instantiation (4 waves x 32 x groups).
 int mfma_block_queries(int fmt, int dm);
 // Resident workgroups per CU of that instantiation (2 if it cannot be queried).
@@ -292,6 +318,11 @@ struct MergeArgs {
   int* widen;
   int* widen_qblocks;
   double* ubound;
+  // With widen set: each widened query is also appended to qmap, its count
+  // in *qmap_n (zeroed before the first merge): the collect launch's packed
+  // blocks.
+  int* qmap;
+  int* qmap_n;
   int widen_all;
 };
 
@@ -337,9 +368,7 @@ void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll
                           int* order, hipStream_t s);
 // *out += number of nonzero entries of f[0, n).
 void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s);
-// list[0 .. *count) = the q < n with flags[q] != 0, in no particular order
-// (*count zeroed here first).
-void launch_pack_flagged(const int* flags, int64_t n, int* list, int* count, hipStream_t s);
+
 // sc->n_uncertified += number of nonzero flags in uncert[0, n).
 void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
                               hipStream_t s);

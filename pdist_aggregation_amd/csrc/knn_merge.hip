@@ -36,6 +36,20 @@ __global__ void fill_keys_kernel(unsigned long long* p, int64_t n, unsigned long
     p[i] = v;
 }
 
+__global__ void fill_batch_kernel(FillBatch b) {
+  const int s = blockIdx.y;
+  const int64_t n = b.n[s];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (b.wide[s]) {
+    unsigned long long* p = static_cast<unsigned long long*>(b.p[s]);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = b.v[s];
+  } else {
+    unsigned* p = static_cast<unsigned*>(b.p[s]);
+    const unsigned v = (unsigned)b.v[s];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+  }
+}
+
 __global__ void fill_u32_kernel(unsigned int* p, int64_t n, unsigned int v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -645,6 +659,7 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
       if ((double)tf < thr) tf = next_up(tf);
       a.widen[q] = 1;
       atomicOr(&a.widen_qblocks[q / a.qblock_queries], 1);
+      a.qmap[atomicAdd(a.qmap_n, 1)] = (int)q;
       a.ubound[q] = kth_s;
       a.tau[q] = ordered_key((double)tf);
     }
@@ -839,6 +854,16 @@ void launch_fill_u32(unsigned int* p, int64_t n, unsigned int v, hipStream_t s) 
   hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, p, n, v);
 }
 
+void launch_fill_batch(FillBatch& b, hipStream_t s) {
+  if (b.count == 0) return;
+  int64_t most = 0;
+  for (int i = 0; i < b.count; ++i) most = b.n[i] > most ? b.n[i] : most;
+  const int64_t blocks = ceil_div(most, kThreads) < 1024 ? ceil_div(most, kThreads) : 1024;
+  hipLaunchKernelGGL(fill_batch_kernel, dim3((unsigned)blocks, (unsigned)b.count), dim3(kThreads), 0,
+                     s, b);
+  b.count = 0;
+}
+
 void launch_sample_guess(const float* sample_out, int64_t n_q, int n_runs, int j,
                          unsigned long long* tau, float* guess, int fail_stride, int* failed,
                          int* failed_qblocks, int qblock_queries, hipStream_t s) {
@@ -957,36 +982,6 @@ void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll
   hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, s, counts);
   hipLaunchKernelGGL(order_scatter_kernel, dim3(g), dim3(kThreads), 0, s, key, n_q, n_t, counts,
                      order);
-}
-
-namespace {
-
-// One atomic per wave reserves the slots of its flagged entries.
-__global__ __launch_bounds__(kThreads) void pack_flagged_kernel(const int* flags, int64_t n,
-                                                                int* list, int* count) {
-  const int lane = threadIdx.x & 63;
-  for (int64_t base = (int64_t)blockIdx.x * kThreads; base < n;
-       base += (int64_t)gridDim.x * kThreads) {
-    const int64_t q = base + threadIdx.x;
-    const bool on = q < n && flags[q] != 0;
-    const uint64_t m = __builtin_amdgcn_ballot_w64(on);
-    if (m == 0) continue;  // uniform across the wave
-    int at = 0;
-    if (lane == 0) at = atomicAdd(count, __popcll(m));
-    at = __shfl(at, 0, 64);
-    const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
-    if (on) list[at + rank] = (int)q;
-  }
-}
-
-}  // namespace
-
-void launch_pack_flagged(const int* flags, int64_t n, int* list, int* count, hipStream_t s) {
-  launch_fill_u32(reinterpret_cast<unsigned*>(count), 1, 0u, s);
-  if (n <= 0) return;
-  const unsigned g = (unsigned)std::min<int64_t>(1024, ceil_div(n, (int64_t)kThreads));
-  hipLaunchKernelGGL(pack_flagged_kernel, dim3(g), dim3(kThreads), 0, s, flags, n, list, count);
 }
 
 void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s) {
