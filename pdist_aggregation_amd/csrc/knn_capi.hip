@@ -656,6 +656,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   fills.add64(tau, n_test, knn::ordered_key(__builtin_huge_val()));
   knn::MergeArgs<T> ma;
   ma.unsorted = 0;
+  ma.stage_cap = 0;
   ma.part_n = nullptr;
   ma.pre_d = nullptr;
   ma.pre_i = nullptr;
@@ -896,6 +897,10 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     ma.part_d = reinterpret_cast<const T*>(fa.part_d);
     ma.part_i = fa.part_i;
     ma.unsorted = 1;  // the MFMA search copies short split lists unsorted
+    // Staging for 2 k' entries (at least 256) instead of every list entry:
+    // 1e4 x 1e6 x 128 BF16, 19 lists x k' = 40: 10 KiB -> 4 KiB of LDS per
+    // merge workgroup.  Widening covers the queries whose ties it drops.
+    ma.stage_cap = knn::next_pow2(std::max(256, 2 * p.kc));
     ma.part_n = fa.part_n;  // and writes only their entries
   }
   // 4. merge + exact re-rank

@@ -261,6 +261,13 @@ struct MergeArgs {
   // split's list as it stands unless it outgrew k'); every entry at or below
   // the query's threshold is then ranked.  0: each list is ascending.
   int unsorted;
+  // Unsorted lists only (0: off): the entries at or below the k'-th value are
+  // staged in LDS for at most stage_cap of them, every one below it first,
+  // then ties at it.  A tie left out has a value equal to the k'-th, and the
+  // certification only assumes every row left out is at or above it, so a
+  // query either certifies or is widened as before; the smaller staging
+  // arrays let more merge workgroups share a CU.
+  int stage_cap;
   // Valid entries per (query, list), [n_q][n_splits] (nullptr: all k' slots
   // are written, the unused ones with +inf / -1).
   const int* part_n;
@@ -411,6 +418,6 @@ void launch_bigk_select_sort(const double* D, int64_t n_q, int64_t n_t, int k, i
                              double* sel_d, int* sel_i, T* dist_out, int64_t* idx_out,
                              hipStream_t s);
 
-size_t merge_lds_bytes(int n_splits, int kc, size_t elem);
+size_t merge_lds_bytes(int n_splits, int kc, size_t elem, int stage_cap);
 
 }  // namespace knn

@@ -303,8 +303,13 @@ __device__ float block_select_kth(const float (&v)[N], const int (&id)[N], int r
 // One workgroup of NT threads per query (final merge) or per (query, group of
 // splits) (pre-merge, a.pre_d set); the launch guarantees lists * k' <= NT *
 // kPerThread.
+// The float one-wave form is held to 80 VGPRs (6 waves per SIMD, 24
+// queries in flight per CU with its capped staging): the merge waits on
+// memory round trips, so more queries in flight is what shortens it.
 template <class T, int NT>
-__global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NT == 64 ? 6
+                                                                                               : 1)))
+void merge_rerank_kernel(MergeArgs<T> a) {
   extern __shared__ __align__(16) unsigned char dyn_smem[];
   const bool pre = a.pre_d != nullptr;
   const int64_t q = pre ? (int64_t)blockIdx.x / a.n_groups : (int64_t)blockIdx.x;
@@ -314,7 +319,11 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
   const int first = grp * lists_alloc;
   const int lists = pre ? min(a.group_size, a.n_splits - first) : a.n_splits;
   const int total = lists * a.kc;
-  const int P = next_pow2(lists_alloc * a.kc);
+  const int P_all = next_pow2(lists_alloc * a.kc);
+  // (float lists only: the MFMA searches write them)
+  const bool capped =
+      sizeof(T) == 4 && a.unsorted && a.stage_cap > 0 && a.stage_cap < P_all;  // uniform
+  const int P = capped ? a.stage_cap : P_all;
   const int KP = next_pow2(a.kc);
   const size_t o_si = align16((size_t)P * sizeof(T));
   const size_t o_ed = align16(o_si + (size_t)P * sizeof(int));
@@ -418,6 +427,8 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
   if (tid == 0) n_s = 0;
   __syncthreads();
   {
+    // Every entry at or below the cut, in as many slots as there are (capped
+    // staging: the first P of them).
     bool keep[kPerThread];
     int at[kPerThread];
 #pragma unroll
@@ -425,13 +436,55 @@ __global__ __launch_bounds__(NT) void merge_rerank_kernel(MergeArgs<T> a) {
     wave_slots(keep, rounds, &n_s, at);
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i)
-      if (keep[i]) {
+      if (keep[i] && at[i] < P) {
         sd[at[i]] = v[i];
         si[at[i]] = id[i];
       }
+    if constexpr (sizeof(T) == 4) {
+      __syncthreads();
+      if (capped && n_s > P) {  // uniform
+        // Overflow (only ties at the cut can cause it: fewer than k' entries
+        // lie below it).  Staged again: the entries below the cut, then the
+        // ties of lowest row index that fit, so the staged entries hold the
+        // k' smallest (value, index) keys.
+        __syncthreads();
+        if (tid == 0) n_s = 0;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i) keep[i] = id[i] >= 0 && v[i] < cut;
+        wave_slots(keep, rounds, &n_s, at);
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i)
+          if (keep[i]) {
+            sd[at[i]] = v[i];
+            si[at[i]] = id[i];
+          }
+        __syncthreads();
+        const int below = n_s;
+        // v and id are not read after the staging: the ties' row indices
+        // replace them (indices >= 0, so their bit order is index order).
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i) {
+          const bool tie = id[i] >= 0 && v[i] == cut;
+          id[i] = tie ? id[i] : -1;
+          v[i] = __int_as_float(tie ? id[i] : 0);
+        }
+        __shared__ unsigned hist_t[256];
+        const int imax = __float_as_int(block_select_kth<NT>(v, id, rounds, P - below, hist_t));
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i) keep[i] = id[i] >= 0 && id[i] <= imax;
+        wave_slots(keep, rounds, &n_s, at);
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i)
+          if (keep[i]) {
+            sd[at[i]] = cut;
+            si[at[i]] = id[i];
+          }
+      }
+    }
   }
   __syncthreads();
-  const int n = n_s;
+  const int n = n_s < P ? n_s : P;
   // The k' best, ascending: ranked into the (still unused) exact-distance
   // arrays, or the head of the sorted staging arrays.
   T* od = n <= kRankMax ? reinterpret_cast<T*>(ed) : sd;
@@ -873,8 +926,10 @@ void launch_sample_guess(const float* sample_out, int64_t n_q, int n_runs, int j
                      failed, failed_qblocks, qblock_queries);
 }
 
-size_t merge_lds_bytes(int n_splits, int kc, size_t elem) {
-  const size_t P = (size_t)next_pow2(n_splits * kc);
+size_t merge_lds_bytes(int n_splits, int kc, size_t elem, int stage_cap) {
+  const int p_all = next_pow2(n_splits * kc);
+  const bool capped = elem == 4 && stage_cap > 0 && stage_cap < p_all;
+  const size_t P = (size_t)(capped ? stage_cap : p_all);
   const size_t KP = (size_t)next_pow2(kc);
   return align16(align16(P * elem) + P * sizeof(int)) + KP * (sizeof(double) + sizeof(int));
 }
@@ -882,7 +937,8 @@ size_t merge_lds_bytes(int n_splits, int kc, size_t elem) {
 template <class T>
 void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s) {
   if (a.n_q <= 0) return;
-  const size_t lds = merge_lds_bytes(a.pre_d ? a.group_size : a.n_splits, a.kc, sizeof(T));
+  const size_t lds = merge_lds_bytes(a.pre_d ? a.group_size : a.n_splits, a.kc, sizeof(T),
+                                     a.unsorted ? a.stage_cap : 0);
   const int64_t blocks = a.pre_d ? a.n_q * a.n_groups : a.n_q;
   const int lists = a.pre_d ? a.group_size : a.n_splits;
   if (lists * a.kc <= 64 * kPerThread)

@@ -18,6 +18,7 @@ import sys
 import numpy as np
 import pytest
 
+import oracle
 import pdist_aggregation_amd as pa
 from knn_checks import EPS_F32, assert_knn_parity, assert_sorted_unique
 
@@ -107,6 +108,31 @@ def test_widen_after_failed_guesses(require_gpu, monkeypatch):
     assert st["n_widened"] == X.shape[0]
     assert st["n_uncertified"] == 0
     assert_knn_parity(X, Y, dist, idx, 10, EPS_F32)
+
+
+@pytest.mark.parametrize("precision", ["exact", "bf16", "fp8"])
+def test_ties_overflowing_the_merge_staging(require_gpu, precision):
+    """600 copies of one row next to query 0: its k'-th candidate value is
+    tied 600 times, more than the merge stages (256), so the staging keeps
+    the ties of lowest row index and the merge's answer is the oracle's (the
+    ten lowest indices of the copies).  The query is widened (its bound does
+    not certify ties at the k-th distance) and its 600 rows at the k-th
+    exact distance exceed the re-rank's 256 staging slots, so it keeps that
+    first answer and is the one query counted as uncertified; every other
+    query matches the oracle."""
+    rs = np.random.RandomState(12)
+    Y = rs.rand(40000, 64).astype(np.float32)
+    X = rs.rand(150, 64).astype(np.float32)
+    rep = np.sort(rs.choice(40000, 600, replace=False))
+    Y[rep] = X[0] + np.float32(0.01)
+    dist, idx, st = _run(Y, X, 10, precision)
+    _say(f"    {precision} staging ties: stats {st}")
+    assert st["n_uncertified"] == 1
+    np.testing.assert_array_equal(idx[0], rep[:10])
+    d0 = oracle._direct_sqdist(X[:1], Y, idx[:1])
+    np.testing.assert_allclose(dist[0], d0[0], rtol=1e-6)
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X[1:], Y, dist[1:], idx[1:], 10, EPS_F32)
 
 
 def test_collect_overflow_keeps_first_answer(require_gpu, monkeypatch):
