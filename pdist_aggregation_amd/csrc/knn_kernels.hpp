@@ -2,9 +2,9 @@
 // kernels.  Every pointer is a device pointer; every launch is asynchronous on
 // `stream`.
 #pragma once
-#include <cstdlib>
 
 #include <hip/hip_runtime.h>
+#include <cstdlib>
 #include <stdint.h>
 
 namespace knn {
@@ -261,12 +261,12 @@ struct MergeArgs {
   // split's list as it stands unless it outgrew k'); every entry at or below
   // the query's threshold is then ranked.  0: each list is ascending.
   int unsorted;
-  // Unsorted lists only (0: off): the entries at or below the k'-th value are
-  // staged in LDS for at most stage_cap of them, every one below it first,
-  // then ties at it.  A tie left out has a value equal to the k'-th, and the
-  // certification only assumes every row left out is at or above it, so a
-  // query either certifies or is widened as before; the smaller staging
-  // arrays let more merge workgroups share a CU.
+  // Unsorted float lists only (0: off): LDS staging for at most stage_cap of
+  // the entries at or below the k'-th value.  Only ties at it can overflow
+  // that (fewer than k' entries lie below it); then the ties of lowest row
+  // index are staged, so the staged entries still hold the k' smallest
+  // (value, index) keys.  The smaller arrays let more merge workgroups share
+  // a CU.
   int stage_cap;
   // Valid entries per (query, list), [n_q][n_splits] (nullptr: all k' slots
   // are written, the unused ones with +inf / -1).
@@ -375,7 +375,6 @@ void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll
                           int* order, hipStream_t s);
 // *out += number of nonzero entries of f[0, n).
 void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s);
-
 // sc->n_uncertified += number of nonzero flags in uncert[0, n).
 void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
                               hipStream_t s);
