@@ -244,6 +244,58 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   int* si = reinterpret_cast<int*>(sd + a.cap);
   const int wq0 = w * WQ;
 
+  // ---- tile loader: global -> LDS (direct) ---------------------------------
+  // Raw buffer loads over this split's rows: rows past the split read as
+  // zero (range-checked by the hardware), so no load needs a branch.  Splits
+  // hold whole tiles except the last one of the train set, whose rows past
+  // n_t take |y|^2 = +inf from the padding after the norms (kNormPad
+  // entries): their values are +inf, never selected.  The look-ahead tiles
+  // past a split are loaded but never used.
+  const int row_bytes = a.ldy * F::esize;
+  const int64_t nrows = yend - ybeg;
+  const i32x4 yrs = make_rsrc(reinterpret_cast<const unsigned char*>(a.Y) +
+                                  ybeg * (int64_t)row_bytes,
+                              (int)(nrows * row_bytes));
+  const i32x4 nrs = make_rsrc(a.ynorm + ybeg, (int)((nrows + 3 * kBY) * 4));
+  i32x4 ers = nrs;
+  if constexpr (FMT == kFmtFP8) ers = make_rsrc(a.yscale + ybeg, (int)nrows);
+  // Load instruction i of wave w fills LDS chunks [(4i + w) * 64, + 64); lane
+  // l brings the chunk for position (4i + w) * 64 + l.  Each tile's loads go
+  // through resources moved to the tile (scalar work only), so a lane's
+  // offsets are the same for every tile.  Float32 features past d come from
+  // an offset past the buffer, i.e. zero.
+  constexpr int kOOB = 0x7ffffff0;
+  int voff[CPT];
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int slot = (i * 4 + w) * 64 + lane;
+    const int r = slot / G::NCH, j = (slot % G::NCH) ^ G::swz(r);
+    voff[i] = (FMT == kFmtF32 && j * 4 >= a.d) ? kOOB : r * row_bytes + j * 16;
+  }
+  const int noff = lane * 4, eoff = lane * 4;
+  // LDS byte address of this wave's first chunk in buffer 0.
+  const unsigned lds_w = lds_address(&tbuf[0].y[0]) + (unsigned)w * 64 * 16;
+  constexpr int kBufBytes = (int)sizeof(TileBuf);
+  constexpr int kNOff = (int)offsetof(TileBuf, n), kEOff = (int)offsetof(TileBuf, e);
+  auto load_tile = [&](int t, auto buf_c) {  // tile t of this split -> tbuf[buf_c]
+    constexpr int B = decltype(buf_c)::value;
+    const int tb = t * kBY;
+    const i32x4 yt = rsrc_advance(yrs, tb * row_bytes);
+    static_for<CPT>([&](auto ic) {
+      constexpr int I = decltype(ic)::value;
+      if ((I * 4 + w) * 64 >= CHUNKS) return;  // wave-uniform
+      lds_load16<B * kBufBytes + I * 4 * 64 * 16>(lds_w, voff[I], yt);
+    });
+    if (w == 0)  // 64 norms: the tile's 32 rows and 32 unused
+      lds_load4<B * kBufBytes + kNOff>(lds_w, noff, rsrc_advance(nrs, tb * 4));
+    if constexpr (FMT == kFmtFP8)
+      if (w == 1)  // 4 scale bytes per lane: the tile's 32 in lanes 0..7
+        lds_load4<B * kBufBytes + kEOff - 1 * 64 * 16>(lds_w, eoff, rsrc_advance(ers, tb));
+  };
+  // The first two tiles are in flight while the queries load.
+  load_tile(0, std::integral_constant<int, 0>());
+  load_tile(1, std::integral_constant<int, 1>());
+
   // ---- resident queries: B operand, scaled by -2 ------------------------------
   typename F::breg xr[QG][NSTEP];
   int xsc[QG];  // FP8: E8M0 scale of -2 x (the row scale times 2)
@@ -313,54 +365,6 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   }
   for (int i = tid; i < BQ; i += kThreads) tau_s[i] = FLT_MAX;
 
-  // ---- tile loader: global -> LDS (direct) ---------------------------------
-  // Raw buffer loads over this split's rows: rows past the split read as
-  // zero (range-checked by the hardware), so no load needs a branch.  Splits
-  // hold whole tiles except the last one of the train set, whose rows past
-  // n_t take |y|^2 = +inf from the padding after the norms (kNormPad
-  // entries): their values are +inf, never selected.  The look-ahead tiles
-  // past a split are loaded but never used.
-  const int row_bytes = a.ldy * F::esize;
-  const int64_t nrows = yend - ybeg;
-  const i32x4 yrs = make_rsrc(reinterpret_cast<const unsigned char*>(a.Y) +
-                                  ybeg * (int64_t)row_bytes,
-                              (int)(nrows * row_bytes));
-  const i32x4 nrs = make_rsrc(a.ynorm + ybeg, (int)((nrows + 3 * kBY) * 4));
-  i32x4 ers = nrs;
-  if constexpr (FMT == kFmtFP8) ers = make_rsrc(a.yscale + ybeg, (int)nrows);
-  // Load instruction i of wave w fills LDS chunks [(4i + w) * 64, + 64); lane
-  // l brings the chunk for position (4i + w) * 64 + l.  Each tile's loads go
-  // through resources moved to the tile (scalar work only), so a lane's
-  // offsets are the same for every tile.  Float32 features past d come from
-  // an offset past the buffer, i.e. zero.
-  constexpr int kOOB = 0x7ffffff0;
-  int voff[CPT];
-#pragma unroll
-  for (int i = 0; i < CPT; ++i) {
-    const int slot = (i * 4 + w) * 64 + lane;
-    const int r = slot / G::NCH, j = (slot % G::NCH) ^ G::swz(r);
-    voff[i] = (FMT == kFmtF32 && j * 4 >= a.d) ? kOOB : r * row_bytes + j * 16;
-  }
-  const int noff = lane * 4, eoff = lane * 4;
-  // LDS byte address of this wave's first chunk in buffer 0.
-  const unsigned lds_w = lds_address(&tbuf[0].y[0]) + (unsigned)w * 64 * 16;
-  constexpr int kBufBytes = (int)sizeof(TileBuf);
-  constexpr int kNOff = (int)offsetof(TileBuf, n), kEOff = (int)offsetof(TileBuf, e);
-  auto load_tile = [&](int t, auto buf_c) {  // tile t of this split -> tbuf[buf_c]
-    constexpr int B = decltype(buf_c)::value;
-    const int tb = t * kBY;
-    const i32x4 yt = rsrc_advance(yrs, tb * row_bytes);
-    static_for<CPT>([&](auto ic) {
-      constexpr int I = decltype(ic)::value;
-      if ((I * 4 + w) * 64 >= CHUNKS) return;  // wave-uniform
-      lds_load16<B * kBufBytes + I * 4 * 64 * 16>(lds_w, voff[I], yt);
-    });
-    if (w == 0)  // 64 norms: the tile's 32 rows and 32 unused
-      lds_load4<B * kBufBytes + kNOff>(lds_w, noff, rsrc_advance(nrs, tb * 4));
-    if constexpr (FMT == kFmtFP8)
-      if (w == 1)  // 4 scale bytes per lane: the tile's 32 in lanes 0..7
-        lds_load4<B * kBufBytes + kEOff - 1 * 64 * 16>(lds_w, eoff, rsrc_advance(ers, tb));
-  };
   // Loads each wave issues per tile, counting only what every wave issues:
   // waiting until at most that many are outstanding completes every load of
   // the tile before (in-order completion), then the barrier makes all waves'
@@ -711,8 +715,6 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     using B1 = std::integral_constant<int, 1>;
     using B2 = std::integral_constant<int, 2>;
     f32x16 acc[QG];
-    load_tile(0, B0());
-    load_tile(1, B1());
     // At step t's wait: n_stores when steps t-2 and t-1 began.
     int st_m2 = 0, st_m1 = 0;
     // Wave priority rotates with the ring cycle (3 steps), offset by the
@@ -802,10 +804,11 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   // threshold is known).  The others go out as they stand, unsorted (the
   // merge ranks every entry), for all queries of the wave at once: entry j
   // of query qq is item qq * k' + j, items are dealt to the lanes, and each
-  // lane has kCopyBatch loads in flight before it stores.  Done one query
-  // after another, each copy waited for its own loads (about 0.4 ms of the
-  // launch at 1e4 x 1e6 x 128).  Only the list's entries are written, and
-  // their count (the merge reads no further).
+  // lane has kCopyBatch loads in flight before it stores (20: two rounds
+  // for k' = 40, the sweep's registers are free by now; 8 cost 1.3 % of the
+  // BF16 search).  Done one query after another, each copy waited for its
+  // own loads (about 0.4 ms of the launch at 1e4 x 1e6 x 128).  Only the
+  // list's entries are written, and their count (the merge reads no further).
 #pragma unroll
   for (int g = 0; g < QG; ++g) cnt_s[2 * (wq0 + g * 32 + c) + h] = cnt[g];
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -826,7 +829,7 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     if (lane == 0) a.part_n[(size_t)q * a.n_splits + split] = a.kc;
     wave_sync();
   }
-  constexpr int kCopyBatch = 8;
+  constexpr int kCopyBatch = 20;
   const int items = WQ * a.kc;
   for (int base = 0; base < items; base += 64 * kCopyBatch) {
     float vd[kCopyBatch];
