@@ -302,13 +302,16 @@ __device__ float block_select_kth(const float (&v)[N], const int (&id)[N], int r
 
 // One workgroup of NT threads per query (final merge) or per (query, group of
 // splits) (pre-merge, a.pre_d set); the launch guarantees lists * k' <= NT *
-// kPerThread.
-// The float one-wave form is held to 80 VGPRs (6 waves per SIMD, 24
-// queries in flight per CU with its capped staging): the merge waits on
-// memory round trips, so more queries in flight is what shortens it.
-template <class T, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && NT == 64 ? 6
-                                                                                               : 1)))
+// PT.
+// PT list entries per thread (the launch picks 12 when a query's lists fit
+// 768 entries, else kPerThread).  The float one-wave forms are held to 72 /
+// 80 VGPRs (7 / 6 waves per SIMD, 28 / 24 queries in flight per CU with the
+// capped staging): the merge waits on memory round trips, so more queries in
+// flight is what shortens it (BF16 1e4 x 1e6 x 128, 19 lists of k' = 40:
+// 0.118 -> 0.112 ms with 12 entries per thread).
+template <class T, int NT, int PT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
+    sizeof(T) == 4 && NT == 64 ? (PT <= 12 ? 7 : 6) : 1)))
 void merge_rerank_kernel(MergeArgs<T> a) {
   extern __shared__ __align__(16) unsigned char dyn_smem[];
   const bool pre = a.pre_d != nullptr;
@@ -351,8 +354,8 @@ void merge_rerank_kernel(MergeArgs<T> a) {
   const int r0 = (a.kc + lists - 1) / lists + 1;
   const int r = a.kc < r0 ? a.kc : r0;
   const bool sorted = !a.unsorted;
-  T v[kPerThread];
-  int id[kPerThread];
+  T v[PT];
+  int id[PT];
   // List of entry e: (e + 0.5) / k' is at least 0.5 / k' from an integer, far
   // more than the float product's error for e < kMaxMerge, so truncation is
   // exact (an integer division per entry cost about 10 % of the merge).
@@ -360,7 +363,7 @@ void merge_rerank_kernel(MergeArgs<T> a) {
   if (tid == 0) n_s = 0;
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kPerThread; ++i) {
+  for (int i = 0; i < PT; ++i) {
     const int e = tid + i * NT;
     bool in = e < total;
     if (in && a.part_n) {  // unwritten slots past the list's entries
@@ -373,16 +376,16 @@ void merge_rerank_kernel(MergeArgs<T> a) {
   }
   const int rounds = (total + NT - 1) / NT;  // uniform
   if (sorted) {
-    bool head[kPerThread];
-    int at[kPerThread];
+    bool head[PT];
+    int at[PT];
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i) {
+    for (int i = 0; i < PT; ++i) {
       const int e = tid + i * NT;
       head[i] = id[i] >= 0 && e % a.kc < r;
     }
     wave_slots(head, rounds, &n_s, at);
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i)
+    for (int i = 0; i < PT; ++i)
       if (head[i]) {
         sd[at[i]] = v[i];
         si[at[i]] = id[i];
@@ -401,7 +404,7 @@ void merge_rerank_kernel(MergeArgs<T> a) {
       __syncthreads();
       int cnt = 0;
 #pragma unroll
-      for (int i = 0; i < kPerThread; ++i) cnt += (i < rounds && id[i] >= 0) ? 1 : 0;
+      for (int i = 0; i < PT; ++i) cnt += (i < rounds && id[i] >= 0) ? 1 : 0;
       if (cnt) atomicAdd(&n_s, cnt);
       __syncthreads();
       const int nv = n_s;
@@ -429,13 +432,13 @@ void merge_rerank_kernel(MergeArgs<T> a) {
   {
     // Every entry at or below the cut, in as many slots as there are (capped
     // staging: the first P of them).
-    bool keep[kPerThread];
-    int at[kPerThread];
+    bool keep[PT];
+    int at[PT];
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i) keep[i] = id[i] >= 0 && v[i] <= cut;
+    for (int i = 0; i < PT; ++i) keep[i] = id[i] >= 0 && v[i] <= cut;
     wave_slots(keep, rounds, &n_s, at);
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i)
+    for (int i = 0; i < PT; ++i)
       if (keep[i] && at[i] < P) {
         sd[at[i]] = v[i];
         si[at[i]] = id[i];
@@ -451,10 +454,10 @@ void merge_rerank_kernel(MergeArgs<T> a) {
         if (tid == 0) n_s = 0;
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i) keep[i] = id[i] >= 0 && v[i] < cut;
+        for (int i = 0; i < PT; ++i) keep[i] = id[i] >= 0 && v[i] < cut;
         wave_slots(keep, rounds, &n_s, at);
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i)
+        for (int i = 0; i < PT; ++i)
           if (keep[i]) {
             sd[at[i]] = v[i];
             si[at[i]] = id[i];
@@ -464,7 +467,7 @@ void merge_rerank_kernel(MergeArgs<T> a) {
         // v and id are not read after the staging: the ties' row indices
         // replace them (indices >= 0, so their bit order is index order).
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i) {
+        for (int i = 0; i < PT; ++i) {
           const bool tie = id[i] >= 0 && v[i] == cut;
           id[i] = tie ? id[i] : -1;
           v[i] = __int_as_float(tie ? id[i] : 0);
@@ -472,10 +475,10 @@ void merge_rerank_kernel(MergeArgs<T> a) {
         __shared__ unsigned hist_t[256];
         const int imax = __float_as_int(block_select_kth<NT>(v, id, rounds, P - below, hist_t));
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i) keep[i] = id[i] >= 0 && id[i] <= imax;
+        for (int i = 0; i < PT; ++i) keep[i] = id[i] >= 0 && id[i] <= imax;
         wave_slots(keep, rounds, &n_s, at);
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i)
+        for (int i = 0; i < PT; ++i)
           if (keep[i]) {
             sd[at[i]] = cut;
             si[at[i]] = id[i];
@@ -941,10 +944,13 @@ void launch_merge_rerank(const MergeArgs<T>& a, hipStream_t s) {
                                      a.unsorted ? a.stage_cap : 0);
   const int64_t blocks = a.pre_d ? a.n_q * a.n_groups : a.n_q;
   const int lists = a.pre_d ? a.group_size : a.n_splits;
-  if (lists * a.kc <= 64 * kPerThread)
-    hipLaunchKernelGGL((merge_rerank_kernel<T, 64>), dim3((unsigned)blocks), dim3(64), lds, s, a);
+  if (lists * a.kc <= 64 * 12)
+    hipLaunchKernelGGL((merge_rerank_kernel<T, 64, 12>), dim3((unsigned)blocks), dim3(64), lds, s, a);
+  else if (lists * a.kc <= 64 * kPerThread)
+    hipLaunchKernelGGL((merge_rerank_kernel<T, 64, kPerThread>), dim3((unsigned)blocks), dim3(64),
+                       lds, s, a);
   else
-    hipLaunchKernelGGL((merge_rerank_kernel<T, kMergeThreads>), dim3((unsigned)blocks),
+    hipLaunchKernelGGL((merge_rerank_kernel<T, kMergeThreads, kPerThread>), dim3((unsigned)blocks),
                        dim3(kMergeThreads), lds, s, a);
 }
 
