@@ -671,8 +671,6 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.widen = nullptr;
   ma.widen_qblocks = nullptr;
   ma.ubound = nullptr;
-  ma.qmap = nullptr;
-  ma.qmap_n = nullptr;
   ma.widen_all = 0;
   knn::MfmaArgs fa;
   ma.xnorm = nullptr;
@@ -777,13 +775,11 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       if (std::strcmp(w, "all") == 0) ma.widen_all = 1;
       if (std::strncmp(w, "every:", 6) == 0) ma.widen_all = std::max(1, std::atoi(w + 6));
     }
-    // The widened queries' packed list for the collect launch (filled by the
-    // merges), its count after the n_test entries.
-    ma.qmap = reinterpret_cast<int*>(ws + p.off_qmap);
-    ma.qmap_n = ma.qmap + n_test;
+    // The widened queries' packed list for the collect launch, its count
+    // after the n_test entries.
     fills.add32(ma.widen, n_test, 0u);
     fills.add32(ma.widen_qblocks, p.n_qblocks, 0u);
-    fills.add32(ma.qmap_n, 1, 0u);
+    fills.add32(reinterpret_cast<int*>(ws + p.off_qmap) + n_test, 1, 0u);
     fills.add32(ws + p.off_coll_cnt, n_test, 0u);
     // |y|^2 = +inf past the last train row (the kernels' partial last tile)
     fills.add32(reinterpret_cast<float*>(ynorm) + n_train, knn::kNormPad, 0x7f800000u);
@@ -1001,12 +997,14 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
         if (p.n_qblocks * p.n_splits <= p.slots)
           ca.coll_group = (int)std::max<int64_t>(1, ca.n_splits / std::max(1, p.n_splits));
       }
-      // The widened queries packed into the first query blocks (listed by
-      // the merges): a few of them take one block's sweep instead of one per
+      // The widened queries packed into the first query blocks (one atomic
+      // per wave): a few of them take one block's sweep instead of one per
       // block they fall in (BF16, 100 Gaussians, 1e5 x 1e6 x 128: 54 widened
       // queries in about 50 blocks).
-      ca.qmap = ma.qmap;
-      ca.qmap_n = ma.qmap_n;
+      int* qmap = reinterpret_cast<int*>(ws + p.off_qmap);
+      knn::launch_pack_flagged(ma.widen, n_test, qmap, qmap + n_test, stream);
+      ca.qmap = qmap;
+      ca.qmap_n = qmap + n_test;
       knn::launch_mfma_collect(ca, stream);
       knn::CollectArgs<T> cr;
       cr.X = X_test;

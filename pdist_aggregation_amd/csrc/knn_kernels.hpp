@@ -325,11 +325,6 @@ struct MergeArgs {
   int* widen;
   int* widen_qblocks;
   double* ubound;
-  // With widen set: each widened query is also appended to qmap, its count
-  // in *qmap_n (zeroed before the first merge): the collect launch's packed
-  // blocks.
-  int* qmap;
-  int* qmap_n;
   int widen_all;
 };
 
@@ -375,6 +370,9 @@ void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll
                           int* order, hipStream_t s);
 // *out += number of nonzero entries of f[0, n).
 void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s);
+// list[*count ...] += the q < n with flags[q] != 0, in no particular order
+// (*count zeroed beforehand).
+void launch_pack_flagged(const int* flags, int64_t n, int* list, int* count, hipStream_t s);
 // sc->n_uncertified += number of nonzero flags in uncert[0, n).
 void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
                               hipStream_t s);
