@@ -71,6 +71,9 @@ typedef struct knn_stats {
                                 row that can lie within their k-th exact distance; such
                                 a query counts in n_uncertified only if that row set
                                 overflowed its capacity */
+  int64_t n_widened_from_lists; /* ... of n_widened, the queries whose row set the
+                                   search's split lists already held (no second sweep
+                                   of the train rows) */
 } knn_stats;
 
 /*

@@ -25,7 +25,7 @@ FLAG_TIMING = 2
 NOALLOC_SCRATCH_BYTES = 128 << 20  # KNN_NOALLOC_SCRATCH_BYTES of include/pdist_knn.h
 KERNEL_PATH = {0: "mfma-16x16x4", 1: "mfma-fp32", 2: "mfma-bf16", 3: "mfma-fp8", 4: "lowd-simt",
                5: "large-k"}
-ABI_VERSION = 3  # knn_abi_version() of the library this binding matches
+ABI_VERSION = 4  # knn_abi_version() of the library this binding matches
 
 # Every symbol include/pdist_knn.h declares.
 EXPORTED = (
@@ -55,6 +55,7 @@ class KnnStats(ctypes.Structure):
         ("merge_ms", ctypes.c_double),
         ("n_rescued", ctypes.c_int64),
         ("n_widened", ctypes.c_int64),
+        ("n_widened_from_lists", ctypes.c_int64),
     ]
 
     def as_dict(self):

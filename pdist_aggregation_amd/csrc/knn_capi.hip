@@ -443,7 +443,18 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     p.n_groups = (int)knn::ceil_div((int64_t)p.n_splits, (int64_t)p.group_size);
   }
   const bool mfma = p.path == kPathMfmaF32 || p.path == kPathMfmaBF16 || p.path == kPathMfmaFP8;
-  if (mfma) plan_guess(p, slots, 2, p.k);
+  if (mfma) {
+    // FP8: deep enough for the split lists to hold the bound-following sets
+    // (about 16 k rows at e4m3 precision: 100-350 at 1e4 x 1e6 x 128 U[0,1)
+    // with k = 10, a whole Gaussian of 2,000 rows for the 1,000-Gaussian
+    // mixture with k = 100), so the widened queries skip the collect sweep:
+    // 4 k' rows, j = 21 and 91 there (2.77 -> 2.05 ms; 65.6 -> about 55 ms).
+    // A guess that falls short of the sets costs more than it saves (mixture,
+    // j = 48: 3 % of the sets in the lists, 71 ms).  Other formats widen few
+    // queries, by about k + 2 rows: planned for k.
+    if (p.path == kPathMfmaFP8) plan_guess(p, slots, 2, 4 * (int64_t)p.kc);
+    if (!p.guess) plan_guess(p, slots, 2, p.k);
+  }
   if (p.path == kPathLowd) plan_guess(p, slots, 1, p.kc);
   // Generic kernel: one run per (query, region), from as many sampling
   // workgroups as the search's own grid target.  Only with 4 or more splits:
@@ -498,7 +509,8 @@ int make_plan(Plan& p, size_t elem, int64_t n_train, int64_t n_test, int64_t dim
     p.coll_cap = std::min(kCollMax, knn::next_pow2(std::max(
                                         kCollMin, (p.path == kPathMfmaFP8 ? 16 : 8) * p.kc)));
     p.off_widen = take((size_t)nq1 * sizeof(int));
-    p.off_widen_qb = take((size_t)p.n_qblocks * sizeof(int));
+    // one flag per query block, then the merge's sink entry
+    p.off_widen_qb = take(((size_t)p.n_qblocks + 1) * sizeof(int));
     p.off_ubound = take((size_t)nq1 * sizeof(double));
     p.off_coll_cnt = take((size_t)nq1 * sizeof(int));
     p.off_coll_idx = take((size_t)nq1 * p.coll_cap * sizeof(int));
@@ -618,6 +630,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       stats->n_uncertified = host_sc.n_uncertified;
       stats->n_rescued = host_sc.n_rescued;
       stats->n_widened = host_sc.n_widened;
+      stats->n_widened_from_lists = host_sc.n_widened_lists;
       if (timer.on) {
         stats->prep_ms = timer.elapsed(0, 1);
         stats->search_ms = timer.elapsed(1, 2);
@@ -672,6 +685,7 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
   ma.widen_qblocks = nullptr;
   ma.ubound = nullptr;
   ma.widen_all = 0;
+  ma.lists_complete = 0;
   knn::MfmaArgs fa;
   ma.xnorm = nullptr;
   ma.xerr = nullptr;
@@ -771,9 +785,13 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     ma.qblock_queries = knn::mfma_block_queries(
         p.path == kPathMfmaF32 ? knn::kFmtF32 : p.path == kPathMfmaBF16 ? knn::kFmtBF16 : knn::kFmtFP8,
         p.dm);
+    // "all" / "every:S" widen every (S-th) query; with "sweep" in the value
+    // every widened query takes the collect launch, none its split lists.
+    bool sweep_only = false;
     if (const char* w = test_switch("PDIST_KNN_WIDEN")) {
-      if (std::strcmp(w, "all") == 0) ma.widen_all = 1;
+      if (std::strncmp(w, "all", 3) == 0) ma.widen_all = 1;
       if (std::strncmp(w, "every:", 6) == 0) ma.widen_all = std::max(1, std::atoi(w + 6));
+      sweep_only = std::strstr(w, "sweep") != nullptr;
     }
     // The widened queries' packed list for the collect launch, its count
     // after the n_test entries.
@@ -801,6 +819,9 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       fa.qslots = reinterpret_cast<float*>(ws + p.off_slots);
       fills.add32(fa.qslots, n_test * fa.slot_stride, 0x7f800000u);  // +inf
     }
+    // A slot bound filters below the shared threshold, so rows under the
+    // latter may be missing from the lists; without slots none are.
+    ma.lists_complete = fa.qslots == nullptr && !sweep_only ? 1 : 0;
     knn::launch_fill_batch(fills, stream);
     if (p.path == kPathMfmaF32) {
       // 2. norms
@@ -1002,7 +1023,13 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
       // block they fall in (BF16, 100 Gaussians, 1e5 x 1e6 x 128: 54 widened
       // queries in about 50 blocks).
       int* qmap = reinterpret_cast<int*>(ws + p.off_qmap);
-      knn::launch_pack_flagged(ma.widen, n_test, qmap, qmap + n_test, stream);
+      knn::launch_pack_flagged(ma.widen, n_test, 1, qmap, qmap + n_test, stream);
+      // Widened queries whose split lists hold the whole set (flag 2: the
+      // collect threshold lies below the search's final one) skip the sweep.
+      if (ma.lists_complete)
+        knn::launch_list_collect(ma.widen, tau, fa.part_d, fa.part_i, fa.part_n, n_test,
+                                 p.n_splits, p.kc, n_train, ca.coll_cnt, ca.coll_idx, p.coll_cap,
+                                 stream);
       ca.qmap = qmap;
       ca.qmap_n = qmap + n_test;
       knn::launch_mfma_collect(ca, stream);
@@ -1034,7 +1061,10 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
         cr.order = order;
       }
       knn::launch_collect_rerank<T>(cr, stream);
-      if (stats) knn::launch_count_nonzero_i32(ma.widen, n_test, &sc->n_widened, stream);
+      if (stats) {
+        knn::launch_count_i32(ma.widen, n_test, 0, &sc->n_widened, stream);
+        knn::launch_count_i32(ma.widen, n_test, 2, &sc->n_widened_lists, stream);
+      }
     }
   }
   // Every query's final merge (the first one, or the re-search's for a query
@@ -1217,6 +1247,6 @@ size_t knn_argkmin_workspace_size(int dtype_bytes, int64_t n_train, int64_t n_te
 
 const char* knn_last_error(void) { return g_last_error.c_str(); }
 
-int knn_abi_version(void) { return 3; }
+int knn_abi_version(void) { return 4; }
 
 }  // extern "C"

@@ -26,7 +26,7 @@ struct DeviceScalars {
   unsigned int n_uncertified;
   unsigned int n_rescued;            // queries searched again after a too-tight sampled threshold
   unsigned int n_widened;            // queries re-ranked over their bound-following row set
-  unsigned int pad_;
+  unsigned int n_widened_lists;      // ... of which the set came from the search's lists
 };
 
 // Everything the fused search kernels need.  T is the element type of the
@@ -106,9 +106,10 @@ struct MfmaArgs {
   // Split s covers rows [s * split_stride, s * split_stride + rows_per_split)
   // (split_stride == rows_per_split for the search; larger for sampling).
   int64_t split_stride;
-  // Re-search launch only: workgroups of query blocks whose qblock_active
-  // entry is 0 exit at once, queries whose query_active entry is 0 neither
-  // collect candidates nor write lists.
+  // Re-search and collect launches only: workgroups of query blocks whose
+  // qblock_active entry is 0 exit at once, queries whose query_active entry
+  // is 0 (collect launch: other than 1) neither collect candidates nor write
+  // lists.
   const int* qblock_active;
   const int* query_active;
   // Sampling launch: [n_q][n_splits][2][kSampleJ] ascending smallest
@@ -316,16 +317,27 @@ struct MergeArgs {
   int* pre_i;
   int group_size, n_groups;
   // Widening (widen == nullptr: off).  A query its final merge does not
-  // certify gets widen[q] = 1, widen_qblocks[q / qblock_queries] = 1,
-  // ubound[q] = its k-th exact squared distance U, and tau[q] = the key of T,
-  // the largest value the search can compute for a row within distance
-  // sqrt(U) of the query: a collect launch then gathers every such row.
+  // certify gets ubound[q] = its k-th exact squared distance U and tau[q] =
+  // the key of T, the largest value the search can compute for a row within
+  // distance sqrt(U) of the query, and then either
+  //  * widen[q] = 2 when lists_complete and T lies below the query's final
+  //    shared threshold: the split lists already hold every row at or below
+  //    T, and launch_list_collect gathers them from there; or
+  //  * widen[q] = 1 and widen_qblocks[q / qblock_queries] = 1: a collect
+  //    launch sweeps the train rows for them.
+  // widen_qblocks holds ceil(n_q / qblock_queries) + 1 entries: the last is
+  // a sink the first kind of query stores into.
+  // lists_complete: the search kept every row below the final tau[q] in its
+  // split lists (no split slots: the sweep filtered with shared thresholds
+  // only, each of which tau[q] ends at or below, and every list cut lowered
+  // tau[q] to the k'-th value it kept).
   // widen_all (testing): S > 0 widens every S-th query (q % S == 0) that is
   // not trivially certified.
   int* widen;
   int* widen_qblocks;
   double* ubound;
   int widen_all;
+  int lists_complete;
 };
 
 template <class T>
@@ -368,11 +380,22 @@ constexpr int kOrderBuckets = 16384;
 void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll_idx,
                           int coll_cap, int64_t n_q, int64_t n_t, int* key, int* counts,
                           int* order, hipStream_t s);
-// *out += number of nonzero entries of f[0, n).
-void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s);
-// list[*count ...] += the q < n with flags[q] != 0, in no particular order
-// (*count zeroed beforehand).
-void launch_pack_flagged(const int* flags, int64_t n, int* list, int* count, hipStream_t s);
+// *out += number of entries of f[0, n) that are nonzero (value 0) or equal
+// to value (otherwise).
+void launch_count_i32(const int* f, int64_t n, int value, unsigned int* out, hipStream_t s);
+// list[*count ...] += the q < n with flags[q] == value, in no particular
+// order (*count zeroed beforehand).
+void launch_pack_flagged(const int* flags, int64_t n, int value, int* list, int* count,
+                         hipStream_t s);
+// Collect lists from the search's split lists, for every query with
+// widen[q] == 2: coll_idx[q * coll_cap ...] = the rows of its n_splits lists
+// (part_d / part_i[(q * n_splits + s) * kc + j], j < part_n[q * n_splits + s])
+// whose value is not above the float threshold of key tau[q], and
+// coll_cnt[q] = their number (written past coll_cap only as a count, like
+// the collect launch's overflow).
+void launch_list_collect(const int* widen, const unsigned long long* tau, const float* part_d,
+                         const int* part_i, const int* part_n, int64_t n_q, int n_splits, int kc,
+                         int64_t n_t, int* coll_cnt, int* coll_idx, int coll_cap, hipStream_t s);
 // sc->n_uncertified += number of nonzero flags in uncert[0, n).
 void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,
                               hipStream_t s);

@@ -713,11 +713,21 @@ void merge_rerank_kernel(MergeArgs<T> a) {
                          1e-300;
       float tf = (float)thr;
       if ((double)tf < thr) tf = next_up(tf);
-      a.widen[q] = 1;
+      // Every row with a value below the search's final threshold sits in
+      // one of the split lists (see lists_complete); rows at or below tf
+      // below it are among them.  (tau[q] is still that threshold: only
+      // this workgroup writes it, below.)
+      const bool from_lists =
+          a.lists_complete && (double)tf < from_ordered_key(a.tau[q]);
+      a.widen[q] = from_lists ? 2 : 1;
       // Plain stores: every writer of a block flag stores 1 (an atomic per
       // widened query on 40 flags and a shared list counter took 19 us per
       // query of the FP8 merge at 1e4 x 1e6 x 128, where every query widens).
-      a.widen_qblocks[q / a.qblock_queries] = 1;
+      // A query taken from its lists stores into the sink entry past the
+      // last block (a conditional store here cost the kernel a private
+      // segment).
+      const int64_t sink = (a.n_q + a.qblock_queries - 1) / a.qblock_queries;
+      a.widen_qblocks[from_lists ? sink : q / a.qblock_queries] = 1;
       a.ubound[q] = kth_s;
       a.tau[q] = ordered_key((double)tf);
     }
@@ -841,14 +851,13 @@ __global__ __launch_bounds__(NT) void collect_rerank_kernel(CollectArgs<T> a) {
   if (tid == 0) a.uncert[q] = 0;
 }
 
-__global__ __launch_bounds__(kThreads) void count_nonzero_i32_kernel(const int* __restrict__ f,
-                                                                     int64_t n,
-                                                                     unsigned int* out) {
+__global__ __launch_bounds__(kThreads) void count_i32_kernel(const int* __restrict__ f, int64_t n,
+                                                             int value, unsigned int* out) {
   __shared__ unsigned int wsum[kThreads / 64];
   unsigned int c = 0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kThreads)
-    c += f[i] != 0;
+    c += value ? f[i] == value : f[i] != 0;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
@@ -1052,12 +1061,13 @@ namespace {
 
 // One atomic per wave reserves the slots of its flagged entries.
 __global__ __launch_bounds__(kThreads) void pack_flagged_kernel(const int* flags, int64_t n,
-                                                                int* list, int* count) {
+                                                                int value, int* list,
+                                                                int* count) {
   const int lane = threadIdx.x & 63;
   for (int64_t base = (int64_t)blockIdx.x * kThreads; base < n;
        base += (int64_t)gridDim.x * kThreads) {
     const int64_t q = base + threadIdx.x;
-    const bool on = q < n && flags[q] != 0;
+    const bool on = q < n && flags[q] == value;
     const uint64_t m = __builtin_amdgcn_ballot_w64(on);
     if (m == 0) continue;  // uniform across the wave
     int at = 0;
@@ -1069,19 +1079,69 @@ __global__ __launch_bounds__(kThreads) void pack_flagged_kernel(const int* flags
   }
 }
 
-}  // namespace
-
-void launch_pack_flagged(const int* flags, int64_t n, int* list, int* count, hipStream_t s) {
-  if (n <= 0) return;
-  const unsigned g = (unsigned)std::min<int64_t>(1024, ceil_div(n, (int64_t)kThreads));
-  hipLaunchKernelGGL(pack_flagged_kernel, dim3(g), dim3(kThreads), 0, s, flags, n, list, count);
+// One wave per query with widen[q] == 2: the entries of its split lists not
+// above its collect threshold, appended in list order (ballot ranks, no
+// atomics).  Splits cover disjoint row ranges, so no row comes twice.
+__global__ __launch_bounds__(kThreads) void list_collect_kernel(
+    const int* __restrict__ widen, const unsigned long long* __restrict__ tau,
+    const float* __restrict__ part_d, const int* __restrict__ part_i,
+    const int* __restrict__ part_n, int64_t n_q, int n_splits, int kc, int64_t n_t,
+    int* __restrict__ coll_cnt, int* __restrict__ coll_idx, int coll_cap) {
+  const int lane = threadIdx.x & 63;
+  const int64_t q = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (q >= n_q || widen[q] != 2) return;  // whole wave
+  const float t = (float)from_ordered_key(tau[q]);
+  const int total = n_splits * kc;
+  const float* pd = part_d + (size_t)q * total;
+  const int* pi = part_i + (size_t)q * total;
+  const int* pn = part_n ? part_n + (size_t)q * n_splits : nullptr;
+  int* out = coll_idx + (size_t)q * coll_cap;
+  int n = 0;
+  for (int e0 = 0; e0 < total; e0 += 64) {
+    const int e = e0 + lane;
+    bool keep = false;
+    int id = -1;
+    if (e < total) {
+      const int l = e / kc;
+      if (!pn || e - l * kc < pn[l]) {
+        id = pi[e];
+        // As the collect launch's test: a NaN value is not above t.
+        keep = !(pd[e] > t) && id >= 0 && id < n_t;
+      }
+    }
+    const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
+    const int at = n + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+    if (keep && at < coll_cap) out[at] = id;
+    n += __popcll(m);
+  }
+  if (lane == 0) coll_cnt[q] = n;
 }
 
-void launch_count_nonzero_i32(const int* f, int64_t n, unsigned int* out, hipStream_t s) {
+}  // namespace
+
+void launch_pack_flagged(const int* flags, int64_t n, int value, int* list, int* count,
+                         hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>(1024, ceil_div(n, (int64_t)kThreads));
+  hipLaunchKernelGGL(pack_flagged_kernel, dim3(g), dim3(kThreads), 0, s, flags, n, value, list,
+                     count);
+}
+
+void launch_list_collect(const int* widen, const unsigned long long* tau, const float* part_d,
+                         const int* part_i, const int* part_n, int64_t n_q, int n_splits, int kc,
+                         int64_t n_t, int* coll_cnt, int* coll_idx, int coll_cap, hipStream_t s) {
+  if (n_q <= 0) return;
+  const int64_t blocks = ceil_div(n_q, (int64_t)(kThreads / 64));
+  hipLaunchKernelGGL(list_collect_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, widen, tau,
+                     part_d, part_i, part_n, n_q, n_splits, kc, n_t, coll_cnt, coll_idx, coll_cap);
+}
+
+void launch_count_i32(const int* f, int64_t n, int value, unsigned int* out, hipStream_t s) {
   if (n <= 0) return;
   const int64_t blocks = std::min<int64_t>(64, ceil_div(n, (int64_t)kThreads));
-  hipLaunchKernelGGL(count_nonzero_i32_kernel, dim3((unsigned)std::max<int64_t>(1, blocks)),
-                     dim3(kThreads), 0, s, f, n, out);
+  hipLaunchKernelGGL(count_i32_kernel, dim3((unsigned)std::max<int64_t>(1, blocks)),
+                     dim3(kThreads), 0, s, f, n, value, out);
 }
 
 void launch_count_uncertified(const unsigned char* uncert, int64_t n, DeviceScalars* sc,

@@ -308,7 +308,10 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   for (int g = 0; g < QG; ++g) {
     const int64_t q = query_of(wq0 + g * 32 + c);
     const bool qok = q >= 0 && q < a.n_q;
-    qact[g] = qok && (!kSelected || a.query_active[q] != 0);
+    // (collect launch: flag 1; flag 2 marks widened queries whose set comes
+    // from the search's lists)
+    qact[g] = qok && (!kSelected || (MODE == kModeCollect ? a.query_active[q] == 1
+                                                          : a.query_active[q] != 0));
     tau[g] = qact[g] ? FLT_MAX : -__builtin_huge_valf();
     if constexpr (FMT == kFmtF32) {
       const float* xrow = reinterpret_cast<const float*>(a.X) + q * a.ldx;
@@ -829,7 +832,12 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
       a.part_d[o] = sd[j];
       a.part_i[o] = si[j];
     }
-    if (lane == 0) a.part_n[(size_t)q * a.n_splits + split] = a.kc;
+    if (lane == 0) {
+      a.part_n[(size_t)q * a.n_splits + split] = a.kc;
+      // As in a compaction: the shared threshold ends at or below the k'-th
+      // value kept, so every row below it stays in some split's list.
+      atomicMin(&a.tau[q], ordered_key((double)sd[a.kc - 1]));
+    }
     wave_sync();
   }
   constexpr int kCopyBatch = 20;

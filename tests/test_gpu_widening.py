@@ -11,7 +11,11 @@ with failing sampled guesses (the re-search's merge widens too), and with a
 collect set larger than its capacity (the query keeps its first answer and
 stays counted as uncertified).  PDIST_KNN_WIDEN=every:S widens every S-th
 query: fewer than half of them run the collect launch in packed query
-blocks, more run it in the search's own blocks.
+blocks, more run it in the search's own blocks.  A widened query whose
+collect threshold lies below the search's final shared threshold takes its
+row set from the search's split lists instead of the collect launch; "sweep"
+in PDIST_KNN_WIDEN sends every widened query through the collect launch, and
+the two routes must give the same answers.
 """
 import sys
 
@@ -61,8 +65,8 @@ def test_widen_every_query_matches_oracle(require_gpu, monkeypatch, precision, d
 def test_widen_some_queries_packed_or_not(require_gpu, monkeypatch, precision, d, stride):
     """Every stride-th query widened: one (stride 1000) or 1/7 of the queries
     (packed blocks), or half of them (the search's blocks); 1,100 queries
-    span several query blocks."""
-    monkeypatch.setenv("PDIST_KNN_WIDEN", f"every:{stride}")
+    span several query blocks.  All of them take the collect launch."""
+    monkeypatch.setenv("PDIST_KNN_WIDEN", f"every:{stride},sweep")
     rs = np.random.RandomState(d + stride)
     Y = rs.rand(30000, d).astype(np.float32)
     X = rs.rand(1100, d).astype(np.float32)
@@ -70,9 +74,40 @@ def test_widen_some_queries_packed_or_not(require_gpu, monkeypatch, precision, d
     dist, idx, st = _run(Y, X, 10, precision)
     _say(f"    stats {st}")
     assert st["n_widened"] >= len(range(0, X.shape[0], stride))
+    assert st["n_widened_from_lists"] == 0
     assert st["n_uncertified"] == 0
     assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
     assert_knn_parity(X, Y, dist, idx, 10, EPS_F32)
+
+
+@pytest.mark.parametrize("precision,d,k", [("exact", 64, 10), ("bf16", 128, 10),
+                                           ("fp8", 128, 10), ("fp8", 256, 40)])
+def test_widened_sets_from_lists_match_the_sweep(require_gpu, monkeypatch, precision, d, k):
+    """A deep sampled guess (j = 24) leaves every row up to about 24 x 32
+    rows deep in the split lists, so widened queries take their row sets
+    from there; the answers equal those of the collect launch bit for bit
+    (the same rows lie within the k-th exact distance either way) and the
+    oracle's."""
+    rs = np.random.RandomState(40 + d + k)
+    Y = rs.rand(60000, d).astype(np.float32)
+    X = rs.rand(700, d).astype(np.float32)
+    monkeypatch.setenv("PDIST_KNN_GUESS", "j:24")
+    out = {}
+    for route in ("all", "all,sweep"):
+        monkeypatch.setenv("PDIST_KNN_WIDEN", route)
+        _say(f"[widen {route}] {precision} d={d} k={k} 60000 x 700, guess j=24")
+        out[route] = _run(Y, X, k, precision)
+        _say(f"    stats {out[route][2]}")
+    dl, il, sl = out["all"]
+    ds, is_, ss = out["all,sweep"]
+    assert sl["n_widened"] == ss["n_widened"] == X.shape[0]
+    assert ss["n_widened_from_lists"] == 0
+    assert sl["n_widened_from_lists"] > 0
+    assert sl["n_uncertified"] == 0 and ss["n_uncertified"] == 0
+    np.testing.assert_array_equal(il, is_)
+    np.testing.assert_array_equal(dl, ds)
+    assert_sorted_unique(il, dl, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dl, il, k, EPS_F32)
 
 
 @pytest.mark.parametrize("precision", ["exact", "bf16", "fp8"])
