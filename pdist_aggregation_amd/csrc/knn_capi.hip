@@ -774,6 +774,9 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
     fa.qmap = nullptr;
     fa.qmap_n = nullptr;
     fa.coll_group = 1;
+    fa.pk_items = 0;
+    fa.pk_splits = 0;
+    fa.pk_grid = 0;
     fa.sample_out = nullptr;
     fa.coll_cnt = nullptr;
     fa.coll_idx = nullptr;
@@ -1010,6 +1013,15 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
                             knn::kMfmaRowStep;
         ca.split_stride = ca.rows_per_split;
         ca.n_splits = (int)knn::ceil_div(n_train, ca.rows_per_split);
+        // Packed (fewer than half the queries listed, so at most pb_max
+        // blocks): the listed blocks share the whole scratch, so one block
+        // sweeps kCollectMinRows-row splits (FP8 at 1e4 x 1e6 x 128: 245
+        // instead of 50 splits for the few queries the lists miss).
+        ca.pk_items = (int)std::min<int64_t>(room / ((int64_t)bq * kCollectCap), INT_MAX / 2);
+        ca.pk_splits = (int)std::min<int64_t>(want, INT_MAX / 2);
+        const int64_t pb_max = knn::ceil_div(knn::ceil_div(n_test, 2), (int64_t)bq);
+        ca.pk_grid = (int)std::min<int64_t>(
+            ca.pk_items, pb_max * std::max<int64_t>(ca.pk_splits, ca.n_splits));
         // Unpacked (most queries widened), the search's own split count when
         // its workgroups fit one round (FP8 at 1e4 x 1e6 x 128: collect
         // 1.28 -> 1.09 ms); with several rounds the short splits balance the

@@ -212,18 +212,37 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
   // search's are (the leading workgroups, dealt to the XCDs as below).
   const int group = MODE == kModeCollect && !qmap && a.coll_group > 1 ? a.coll_group : 1;
   int qb, split, nsplit = 1;
+  int nqb = a.n_qblocks;  // query blocks of this launch's layout
+  int64_t stride = a.split_stride, rows = a.rows_per_split;
   {
     const int b = blockIdx.x;
-    const int groups = (a.n_splits + group - 1) / group;
-    const int items = a.n_qblocks * groups;
-    const int grid = group == 1 ? (int)gridDim.x : (items + 7) & ~7;
-    if (b >= grid) return;  // past the grouped grid (whole workgroup)
+    int nsp = a.n_splits;
+    if (MODE == kModeCollect && qmap) {
+      // Packed: only the blocks of listed queries, each over as many splits
+      // as the collect scratch holds (pk_items blocks' lists), down to
+      // pk_splits of them, never fewer than the unpacked layout's: a few
+      // listed queries cost a short sweep.  n_mapped is the same for every
+      // workgroup, so all take the same layout (the host sizes the grid for
+      // the largest).
+      nqb = (int)((n_mapped + BQ - 1) / BQ);
+      if (nqb == 0) return;
+      int sp = a.pk_items / nqb;
+      sp = sp < a.pk_splits ? sp : a.pk_splits;
+      sp = sp > a.n_splits ? sp : a.n_splits;
+      rows = ((a.n_t + sp - 1) / sp + kMfmaRowStep - 1) / kMfmaRowStep * kMfmaRowStep;
+      stride = rows;
+      nsp = (int)((a.n_t + rows - 1) / rows);
+    }
+    const int groups = (nsp + group - 1) / group;
+    const int items = nqb * groups;
+    const int grid = (items + 7) & ~7;  // the launch's own grid, or a part of it
+    if (b >= grid) return;  // past this layout's grid (whole workgroup)
     const int per_xcd = grid >> 3;
     const int w_item = (b & 7) * per_xcd + (b >> 3);
     if (w_item >= items) return;  // grid padding (whole workgroup)
-    split = (w_item / a.n_qblocks) * group;
-    qb = w_item % a.n_qblocks;
-    nsplit = a.n_splits - split < group ? a.n_splits - split : group;
+    split = (w_item / nqb) * group;
+    qb = w_item % nqb;
+    nsplit = nsp - split < group ? nsp - split : group;
   }
   constexpr bool kSelected = MODE == kModeResearch || MODE == kModeCollect;
   const int64_t q0 = (int64_t)qb * BQ;
@@ -234,10 +253,10 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     return q0 + ql < n_mapped ? (int64_t)qmap[q0 + ql] : -1;
   };
   // A group of splits is contiguous (split_stride == rows_per_split there).
-  const int64_t ybeg = (int64_t)split * a.split_stride;
-  const int64_t ylen = (int64_t)(nsplit - 1) * a.split_stride + a.rows_per_split;
+  const int64_t ybeg = (int64_t)split * stride;
+  const int64_t ylen = (int64_t)(nsplit - 1) * stride + rows;
   const int64_t yend = ybeg + ylen < a.n_t ? ybeg + ylen : a.n_t;
-  const size_t blk = (size_t)split * a.n_qblocks + qb;
+  const size_t blk = (size_t)split * nqb + qb;
   // Candidate lists of the search launches: (value bits, row) pairs, one
   // 8-byte store per survivor (two 4-byte stores cost about 0.13 ms more at
   // 1e4 x 1e6 x 128).  The collect launch keeps row indices only, in cand_i.
@@ -876,7 +895,9 @@ template <int FMT, int DM, int QG, int MODE>
 void launch_one(const MfmaArgs& a, hipStream_t s) {
   // the list sorts' LDS (collect lists are never sorted in the kernel)
   const size_t lds = MODE == kModeSample || MODE == kModeCollect ? 0 : (size_t)4 * a.cap * 8;
-  const unsigned blocks = (unsigned)((a.n_qblocks * a.n_splits + 7) / 8 * 8);
+  int64_t items = (int64_t)a.n_qblocks * a.n_splits;
+  if (MODE == kModeCollect && a.qmap && a.pk_grid > items) items = a.pk_grid;
+  const unsigned blocks = (unsigned)((items + 7) / 8 * 8);
   hipLaunchKernelGGL((mfma_search_kernel<FMT, DM, QG, MODE>), dim3(blocks), dim3(kThreads), lds,
                      s, a);
 }

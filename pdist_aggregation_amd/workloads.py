@@ -118,7 +118,8 @@ def run(name, reps=3):
             print(f"{name}: call {1e3 * (t1 - t0):.3f} ms (prep {st.prep_ms:.3f}, search "
                   f"{st.search_ms:.3f} = {rate:.1f} TFLOP/s, merge {st.merge_ms:.3f}); "
                   f"path {st.kernel_path}, splits {st.n_splits}, k' {st.k_candidates}, "
-                  f"uncertified {st.n_uncertified}, widened {st.n_widened}, "
+                  f"uncertified {st.n_uncertified}, widened {st.n_widened} "
+                  f"({st.n_widened_from_lists} from the split lists), "
                   f"re-searched {st.n_rescued}", flush=True)
     del ws
     if name == "grid-f64-d500":
