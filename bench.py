@@ -319,6 +319,7 @@ def main():
                "stages_ms": stages, "kernel_path": _capi.KERNEL_PATH.get(st["kernel_path"]),
                "k_candidates": st["k_candidates"], "n_splits": st["n_splits"],
                "n_uncertified": st["n_uncertified"], "n_widened": st.get("n_widened"),
+               "n_widened_from_lists": st.get("n_widened_from_lists"),
                "n_rescued": st["n_rescued"]}
         if ref_idx is not None:
             got_i, got_d = run.idx.cpu().numpy(), run.dist.cpu().numpy()

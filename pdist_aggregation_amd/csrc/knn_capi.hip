@@ -250,6 +250,8 @@ constexpr int kCollectCap = 64;
 // runs in cluster order.
 constexpr int kOrderMinColl = 4096;
 constexpr int64_t kCollectMinRows = 4096;
+// Packed collect launches (a few listed query blocks): shortest split.
+constexpr int64_t kPackedMinRows = 1024;
 
 // runs: sampled runs per (query, region) the kernel writes (2 for the MFMA
 // kernel's two lane halves, 1 for the low-d kernel's one lane per query).
@@ -1015,10 +1017,10 @@ int run(const T* X_train, int64_t n_train, const T* X_test, int64_t n_test, int6
         ca.n_splits = (int)knn::ceil_div(n_train, ca.rows_per_split);
         // Packed (fewer than half the queries listed, so at most pb_max
         // blocks): the listed blocks share the whole scratch, so one block
-        // sweeps kCollectMinRows-row splits (FP8 at 1e4 x 1e6 x 128: 245
+        // sweeps kPackedMinRows-row splits (FP8 at 1e4 x 1e6 x 128: 977
         // instead of 50 splits for the few queries the lists miss).
         ca.pk_items = (int)std::min<int64_t>(room / ((int64_t)bq * kCollectCap), INT_MAX / 2);
-        ca.pk_splits = (int)std::min<int64_t>(want, INT_MAX / 2);
+        ca.pk_splits = (int)std::min<int64_t>(knn::ceil_div(n_train, kPackedMinRows), INT_MAX / 2);
         const int64_t pb_max = knn::ceil_div(knn::ceil_div(n_test, 2), (int64_t)bq);
         ca.pk_grid = (int)std::min<int64_t>(
             ca.pk_items, pb_max * std::max<int64_t>(ca.pk_splits, ca.n_splits));

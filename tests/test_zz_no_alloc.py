@@ -76,7 +76,8 @@ def test_noalloc_matches_workspace_entry_point(require_gpu, precision, d, k, fir
         d1, i1, s1 = _call(torch, lib, Y, X, k, precision, noalloc=False)
         np.testing.assert_array_equal(i0, i1)
         np.testing.assert_array_equal(d0, d1)
-        for key in ("n_uncertified", "n_widened", "n_rescued", "kernel_path", "n_splits"):
+        for key in ("n_uncertified", "n_widened", "n_widened_from_lists", "n_rescued", "kernel_path",
+                    "n_splits"):
             assert s0[key] == s1[key], (key, s0, s1)
         assert s0["n_uncertified"] == 0
         assert_knn_parity(X, Y, d0, i0, k, EPS_F32)
