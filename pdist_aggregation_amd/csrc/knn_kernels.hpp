@@ -137,11 +137,12 @@ struct MfmaArgs {
   // each (1: one split); split_stride must equal rows_per_split then.
   int coll_group;
   // Collect launch, packed: the P = ceil(*qmap_n / BQ) blocks each sweep
-  // S = max(min(pk_items / P, pk_splits), n_splits) splits of equal whole
-  // row steps (pk_items: the block lists the candidate scratch holds, at
-  // least n_qblocks * n_splits), from a grid of at least pk_grid workgroups
-  // (pk_grid >= P * S for every packed P).
-  int pk_items, pk_splits, pk_grid;
+  // S = max(min(pk_items / P, pk_slots / P, pk_splits), n_splits) splits of
+  // equal whole row steps (pk_items: the block lists the candidate scratch
+  // holds, at least n_qblocks * n_splits; pk_slots: workgroups resident at
+  // once), from a grid of at least pk_grid workgroups (pk_grid >= P * S for
+  // every packed P).
+  int pk_items, pk_splits, pk_slots, pk_grid;
 };
 
 constexpr int kSlotJ = 4;

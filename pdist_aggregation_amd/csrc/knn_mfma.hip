@@ -219,15 +219,18 @@ __global__ __launch_bounds__(kThreads, FMT != kFmtF32 ? 2 : DM > 128 ? 1 : 3) vo
     int nsp = a.n_splits;
     if (MODE == kModeCollect && qmap) {
       // Packed: only the blocks of listed queries, each over as many splits
-      // as the collect scratch holds (pk_items blocks' lists), down to
-      // pk_splits of them, never fewer than the unpacked layout's: a few
-      // listed queries cost a short sweep.  n_mapped is the same for every
+      // as the collect scratch holds (pk_items blocks' lists) and one round
+      // of workgroups takes (pk_slots), down to pk_splits of them, never
+      // fewer than the unpacked layout's: a few listed queries cost a short
+      // sweep.  n_mapped is the same for every
       // workgroup, so all take the same layout (the host sizes the grid for
       // the largest).
       nqb = (int)((n_mapped + BQ - 1) / BQ);
       if (nqb == 0) return;
       int sp = a.pk_items / nqb;
       sp = sp < a.pk_splits ? sp : a.pk_splits;
+      const int one_round = a.pk_slots / nqb;  // more splits would queue a second round
+      sp = sp < one_round ? sp : one_round;
       sp = sp > a.n_splits ? sp : a.n_splits;
       rows = ((a.n_t + sp - 1) / sp + kMfmaRowStep - 1) / kMfmaRowStep * kMfmaRowStep;
       stride = rows;
