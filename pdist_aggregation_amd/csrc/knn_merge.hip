@@ -1097,23 +1097,44 @@ __global__ __launch_bounds__(kThreads) void list_collect_kernel(
   const int* pn = part_n ? part_n + (size_t)q * n_splits : nullptr;
   int* out = coll_idx + (size_t)q * coll_cap;
   int n = 0;
-  for (int e0 = 0; e0 < total; e0 += 64) {
-    const int e = e0 + lane;
-    bool keep = false;
-    int id = -1;
-    if (e < total) {
-      const int l = e / kc;
-      if (!pn || e - l * kc < pn[l]) {
-        id = pi[e];
-        // As the collect launch's test: a NaN value is not above t.
-        keep = !(pd[e] > t) && id >= 0 && id < n_t;
-      }
-    }
+  // As the collect launch's test: a NaN value is not above t.
+  auto append = [&](bool keep, int id) {
     const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
     const int at = n + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
     if (keep && at < coll_cap) out[at] = id;
     n += __popcll(m);
+  };
+  if (kc >= 64 && pn) {
+    // Long lists: list by list up to each one's entry count (FP8 on 1,000
+    // Gaussians: about 190 of 400 slots filled per list).
+    for (int l = 0; l < n_splits; ++l) {
+      const int len = __builtin_amdgcn_readfirstlane(pn[l]);
+      for (int j = lane; j - lane < len; j += 64) {
+        bool keep = false;
+        int id = -1;
+        if (j < len) {
+          id = pi[l * kc + j];
+          keep = !(pd[l * kc + j] > t) && id >= 0 && id < n_t;
+        }
+        append(keep, id);
+      }
+    }
+  } else {
+    // Short lists: all slots of all lists in one flat sweep.
+    for (int e0 = 0; e0 < total; e0 += 64) {
+      const int e = e0 + lane;
+      bool keep = false;
+      int id = -1;
+      if (e < total) {
+        const int l = e / kc;
+        if (!pn || e - l * kc < pn[l]) {
+          id = pi[e];
+          keep = !(pd[e] > t) && id >= 0 && id < n_t;
+        }
+      }
+      append(keep, id);
+    }
   }
   if (lane == 0) coll_cnt[q] = n;
 }
