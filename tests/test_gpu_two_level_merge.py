@@ -2,7 +2,8 @@
 workgroup takes (n_splits * k' > 2048), groups of split lists are first
 reduced to one sorted list each, then merged.  PDIST_KNN_SPLITS forces the
 split count (clamped to what two levels take), so the pre-merge runs at test
-sizes; results are compared with the FP64 oracle."""
+sizes; results are compared with the FP64 oracle, also with every query
+widened and its set taken from the split lists."""
 import sys
 
 import numpy as np
@@ -70,3 +71,34 @@ def test_forced_splits_with_sampled_guess_and_failures(require_gpu, monkeypatch)
     assert st["n_rescued"] == len(range(0, 333, 3))
     assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
     assert_knn_parity(X, Y, dist, idx, 20, EPS_F32)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_forced_splits_widened_from_lists(require_gpu, monkeypatch, precision):
+    """Two-level merge with every query widened and a deep sampled guess
+    (j = 24): the widened sets come from the search's own split lists (the
+    pre-merge reduces copies of them) and give the collect sweep's answers
+    bit for bit, and the oracle's."""
+    monkeypatch.setenv("PDIST_KNN_SPLITS", "120")
+    monkeypatch.setenv("PDIST_KNN_GUESS", "j:24")
+    rs = np.random.RandomState(91)
+    Y = rs.rand(60000, 128).astype(np.float32)
+    X = rs.rand(200, 128).astype(np.float32)
+    out = {}
+    for route in ("all", "all,sweep"):
+        monkeypatch.setenv("PDIST_KNN_WIDEN", route)
+        _say(f"[two-level merge, widen {route}] {precision} 200 x 60000 x 128 k=10 splits=120")
+        (dist, idx), _, st = pa.parallel_knn(Y, X, 10, return_distance=True, squared=True,
+                                             precision=precision, return_stats=True)
+        _say(f"    stats {st}")
+        out[route] = (dist, idx, st)
+    dl, il, sl = out["all"]
+    ds, is_, ss = out["all,sweep"]
+    assert sl["n_splits"] * sl["k_candidates"] > 2048, sl
+    assert sl["n_widened"] == ss["n_widened"] == X.shape[0]
+    assert sl["n_widened_from_lists"] > 0 and ss["n_widened_from_lists"] == 0
+    assert sl["n_uncertified"] == 0 and ss["n_uncertified"] == 0
+    np.testing.assert_array_equal(il, is_)
+    np.testing.assert_array_equal(dl, ds)
+    assert_sorted_unique(il, dl, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dl, il, 10, EPS_F32)
