@@ -382,7 +382,8 @@ template <class T>
 void launch_collect_rerank(const CollectArgs<T>& a, hipStream_t s);
 // order[0, n_q): the queries bucketed by the smallest row index each widened
 // query collected (queries whose sets overlap mostly share it), widened ones
-// first.  key, counts: scratch of n_q and kOrderBuckets + 1 ints.
+// first.  key, counts: scratch of n_q and kOrderBuckets + 1 ints (the last
+// holds the largest key in use).
 constexpr int kOrderBuckets = 16384;
 void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll_idx,
                           int coll_cap, int64_t n_q, int64_t n_t, int* key, int* counts,

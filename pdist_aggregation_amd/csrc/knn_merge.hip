@@ -996,14 +996,40 @@ __global__ __launch_bounds__(kThreads) void collect_key_kernel(const int* widen,
   if (lane == 0) key[q] = m < 0 ? 0 : m;
 }
 
-__device__ inline int order_bucket(int key, int64_t n_t) {
-  return (int)((int64_t)key * kOrderBuckets / (n_t + 1));  // key = n_t -> last bucket or below
+// Buckets spread over the keys in use, [0, kmax]: the smallest row index of
+// a set is small whatever the set (1,000 Gaussians over 2e6 rows: all below
+// about 10,000), so buckets over all n_t rows would hold a dozen clusters
+// each.  Keys n_t (not widened) take the last bucket.
+__device__ inline int order_bucket(int key, int64_t n_t, int kmax) {
+  if (key >= n_t) return kOrderBuckets - 1;
+  return (int)((int64_t)key * (kOrderBuckets - 1) / ((int64_t)kmax + 1));
 }
 
-__global__ void order_count_kernel(const int* key, int64_t n_q, int64_t n_t, int* counts) {
+// *kmax = the largest key below n_t (0 if none): one workgroup.
+__global__ __launch_bounds__(1024) void order_max_kernel(const int* key, int64_t n_q, int64_t n_t,
+                                                         int* kmax) {
+  __shared__ int part[1024 / 64];
+  int m = 0;
+  for (int64_t q = threadIdx.x; q < n_q; q += 1024) {
+    const int k = key[q];
+    if (k < n_t && k > m) m = k;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < 1024 / 64; ++i) m = max(m, part[i]);
+    *kmax = m;
+  }
+}
+
+__global__ void order_count_kernel(const int* key, int64_t n_q, int64_t n_t, const int* kmax,
+                                   int* counts) {
+  const int km = *kmax;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_q;
        q += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&counts[order_bucket(key[q], n_t)], 1);
+    atomicAdd(&counts[order_bucket(key[q], n_t, km)], 1);
 }
 
 // Exclusive prefix sum of counts[0, kOrderBuckets) in place, one workgroup of
@@ -1034,11 +1060,12 @@ __global__ __launch_bounds__(1024) void order_scan_kernel(int* counts) {
   }
 }
 
-__global__ void order_scatter_kernel(const int* key, int64_t n_q, int64_t n_t, int* offsets,
-                                     int* order) {
+__global__ void order_scatter_kernel(const int* key, int64_t n_q, int64_t n_t, const int* kmax,
+                                     int* offsets, int* order) {
+  const int km = *kmax;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_q;
        q += (int64_t)gridDim.x * blockDim.x)
-    order[atomicAdd(&offsets[order_bucket(key[q], n_t)], 1)] = (int)q;
+    order[atomicAdd(&offsets[order_bucket(key[q], n_t, km)], 1)] = (int)q;
 }
 
 }  // namespace
@@ -1049,12 +1076,15 @@ void launch_collect_order(const int* widen, const int* coll_cnt, const int* coll
   if (n_q <= 0) return;
   hipLaunchKernelGGL(collect_key_kernel, dim3((unsigned)ceil_div(n_q, kThreads / 64)),
                      dim3(kThreads), 0, s, widen, coll_cnt, coll_idx, coll_cap, n_q, n_t, key);
-  launch_fill_u32(reinterpret_cast<unsigned*>(counts), kOrderBuckets + 1, 0u, s);
+  launch_fill_u32(reinterpret_cast<unsigned*>(counts), kOrderBuckets, 0u, s);
+  int* kmax = counts + kOrderBuckets;  // the scan leaves this entry alone
+  hipLaunchKernelGGL(order_max_kernel, dim3(1), dim3(1024), 0, s, key, n_q, n_t, kmax);
   const unsigned g = (unsigned)std::min<int64_t>(1024, ceil_div(n_q, (int64_t)kThreads));
-  hipLaunchKernelGGL(order_count_kernel, dim3(g), dim3(kThreads), 0, s, key, n_q, n_t, counts);
+  hipLaunchKernelGGL(order_count_kernel, dim3(g), dim3(kThreads), 0, s, key, n_q, n_t, kmax,
+                     counts);
   hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, s, counts);
-  hipLaunchKernelGGL(order_scatter_kernel, dim3(g), dim3(kThreads), 0, s, key, n_q, n_t, counts,
-                     order);
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(g), dim3(kThreads), 0, s, key, n_q, n_t, kmax,
+                     counts, order);
 }
 
 namespace {

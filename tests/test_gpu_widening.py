@@ -189,3 +189,24 @@ def test_collect_overflow_keeps_first_answer(require_gpu, monkeypatch):
     np.testing.assert_array_equal(idx[0], rep[:10])
     assert np.all(dist[0] == 0)
     assert_knn_parity(X, Y, dist, idx, 10, EPS_F32)
+
+
+def test_cluster_ordered_rerank_matches_oracle(require_gpu, monkeypatch):
+    """FP8 with k' >= 256 (collect lists of 4,096 rows and more) re-ranks the
+    widened queries in cluster order: keys (the smallest row index each
+    collected) are bucketed over the range in use, so queries of one cluster
+    run side by side.  24 Gaussian clusters, every query widened: the
+    answers are the oracle's."""
+    rs = np.random.RandomState(5)
+    centres = rs.uniform(-1, 1, (24, 64)).astype(np.float32)
+    Y = (centres[rs.randint(0, 24, 40000)] + 0.05 * rs.randn(40000, 64)).astype(np.float32)
+    X = (centres[rs.randint(0, 24, 300)] + 0.05 * rs.randn(300, 64)).astype(np.float32)
+    k = 64  # k' = 256: FP8 collect lists of 4,096 rows
+    monkeypatch.setenv("PDIST_KNN_WIDEN", "all")
+    _say(f"[widen all] fp8 d=64 k={k} 40000 x 300, 24 clusters")
+    dist, idx, st = _run(Y, X, k, "fp8")
+    _say(f"    stats {st}")
+    assert st["n_widened"] == X.shape[0]
+    assert st["n_uncertified"] == 0
+    assert_sorted_unique(idx, dist, Y.shape[0], X, Y)
+    assert_knn_parity(X, Y, dist, idx, k, EPS_F32)
