@@ -24,6 +24,18 @@ def test_header_and_binding_agree():
     assert declared_functions() == sorted(_capi.EXPORTED)
 
 
+def test_stats_struct_matches_header():
+    """knn_stats: the ctypes mirror has the header's fields in the header's
+    order, with the same widths (int64_t / double), so a library and binding
+    of one ABI version read the same bytes."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    body = re.search(r"typedef struct knn_stats \{(.*?)\} knn_stats;", text, re.S).group(1)
+    fields = re.findall(r"\b(int64_t|double)\s+(\w+)\s*;", body)
+    ctype = {"int64_t": ctypes.c_int64, "double": ctypes.c_double}
+    assert [(n, ctype[t]) for t, n in fields] == list(_capi.KnnStats._fields_)
+    assert ctypes.sizeof(_capi.KnnStats) == 8 * len(fields)
+
+
 def test_library_exports_every_declared_symbol():
     lib = _capi.lib()
     for name in declared_functions():
